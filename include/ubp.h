@@ -1,0 +1,166 @@
+/* ubp.h -- C ABI of the MI355X WCSPH step engine (libubp_hip.so).
+ *
+ * This is the drop-in boundary for the reference's `SimulationKernel` backend
+ * slot.  Every entry point names the reference interface it replaces (paths
+ * relative to the reference repository).  Plain pointers and sizes only; all
+ * particle arrays are host memory in ORIGINAL particle order (the engine
+ * reorders internally and un-permutes on readback).
+ *
+ * Conventions
+ *   - Every call returns 0 on success and a negative UBP_E* code on failure;
+ *     ubp_last_error() returns the calling thread's last error message.
+ *   - A handle may move between threads but must be used by one thread at a
+ *     time (the reference GpuKernel is Send + !Sync,
+ *     backend/crates/kernel/src/gpu/mod.rs:975-978).  Every entry point makes
+ *     the handle's device current on the calling thread; one HIP stream per
+ *     handle.
+ */
+#ifndef UBP_H
+#define UBP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UBP_OK 0
+#define UBP_EINVAL (-1)   /* bad argument / state */
+#define UBP_EHIP (-2)     /* HIP runtime failure */
+#define UBP_ENODEV (-3)   /* no usable gfx950 device */
+#define UBP_ENOMEM (-4)   /* device allocation failed */
+#define UBP_EOVERFLOW (-5)/* a size limit was exceeded */
+
+typedef struct ubp_engine ubp_engine;
+
+/* Input particle set, ParticleArrays layout
+ * (backend/crates/kernel/src/particle.rs:22-58).  NULL vx/vy/vz/ax/ay/az
+ * mean zero; NULL temperature means 293.15 K (particle.rs:94); NULL
+ * fluid_type means Water (0). */
+typedef struct {
+  size_t n;
+  const float *x, *y, *z;
+  const float *vx, *vy, *vz;
+  const float *ax, *ay, *az;
+  const float *density, *mass, *temperature;
+  const uint8_t *fluid_type; /* FluidType repr(u8): 0 Water, 1 Air (particle.rs:9-14) */
+} ubp_particles_in;
+
+/* Static boundary particles, BoundaryParticles layout
+ * (backend/crates/kernel/src/boundary.rs:20-37). */
+typedef struct {
+  size_t n;
+  const float *x, *y, *z, *mass, *nx, *ny, *nz;
+} ubp_boundary_in;
+
+/* Caller-owned output buffers for ubp_read_particles; any NULL is skipped. */
+typedef struct {
+  float *x, *y, *z, *vx, *vy, *vz, *ax, *ay, *az;
+  float *density, *pressure, *mass, *temperature;
+  uint8_t *fluid_type;
+} ubp_particles_out;
+
+/* ErrorMetrics (backend/crates/kernel/src/lib.rs:38-47). */
+typedef struct {
+  float max_density_variation;
+  float energy_conservation;
+  float mass_conservation;
+} ubp_error_metrics_t;
+
+/* GpuStepProfile (backend/crates/kernel/src/gpu/mod.rs:35-52), microseconds,
+ * measured with HIP events on the handle's stream. */
+typedef struct {
+  uint64_t grid_build_us;
+  uint64_t density_us;
+  uint64_t boundary_pressure_us;
+  uint64_t forces_us;
+  uint64_t integrate_us;
+  uint64_t readback_us;
+  uint64_t total_us;
+} ubp_step_profile;
+
+/* Replaces GpuKernel::new (backend/crates/kernel/src/gpu/mod.rs:190-705) and
+ * CpuKernel::new (backend/crates/kernel/src/lib.rs:138-174): the same nine
+ * inputs.  `viscosity` is accepted and ignored like the reference
+ * (lib.rs:107-109); Monaghan alpha/beta default to the reference's 1 and 2
+ * (sph.rs:440-441) and are changed with ubp_set_viscosity_coeffs.  Returns
+ * NULL on failure (see ubp_last_error), the GpuInitError path
+ * (gpu/mod.rs:158-168). */
+ubp_engine *ubp_create(const ubp_particles_in *particles, const ubp_boundary_in *boundary,
+                       float h, const float gravity[3], float speed_of_sound,
+                       float cfl_number, float viscosity, const float domain_min[3],
+                       const float domain_max[3]);
+
+/* Drop for GpuKernel. */
+void ubp_destroy(ubp_engine *e);
+
+/* Thread-local message of the last failed call ("" if none). */
+const char *ubp_last_error(void);
+
+/* Replaces gpu_available() (backend/crates/kernel/src/gpu/mod.rs:171-182):
+ * 1 when a gfx950 device is visible, else 0. */
+int ubp_gpu_available(void);
+
+/* Extension setters (no reference counterpart; must precede the first step).
+ * Monaghan coefficients (the reference hard-codes 1 and 2, sph.rs:440-441). */
+int ubp_set_viscosity_coeffs(ubp_engine *e, float alpha, float beta);
+/* SDF wall: GridSDF node grid (backend/crates/orchestrator/src/geometry.rs:
+ * 11-20; index i*ny*nz + j*nz + k, z fastest), trilinear lookup
+ * (geometry.rs:99-150) and central-difference normal (geometry.rs:153-175). */
+int ubp_set_sdf(ubp_engine *e, const float origin[3], float cell, const uint32_t dims[3],
+                const float *distances);
+/* Neighbour-list capacity and launch shapes (supersedes set_workgroup_size,
+ * backend/crates/kernel/src/gpu/mod.rs:711-767).  0 keeps a value. */
+int ubp_set_tuning(ubp_engine *e, uint32_t block_size, uint32_t nbr_capacity);
+
+/* SimulationKernel::step (backend/crates/kernel/src/lib.rs:60, CPU body
+ * lib.rs:287-335; GPU slot gpu/mod.rs:1642-1693): one KDK step of size dt,
+ * bootstrap forces on the first call, synchronous. */
+int ubp_step(ubp_engine *e, float dt);
+/* GpuKernel::step_no_sync / sync (backend/crates/kernel/src/gpu/mod.rs:
+ * 996-1036): enqueue without waiting; ubp_sync waits for the stream. */
+int ubp_step_async(ubp_engine *e, float dt);
+int ubp_sync(ubp_engine *e);
+/* New: n steps with dt = compute_timestep(state) (sph.rs:591-628) evaluated on
+ * the device before each step (the runner loop of orchestrator/src/runner.rs:
+ * 208-238 without host readback).  Asynchronous; ubp_sync to wait.  When
+ * `use_graph` is non-zero the steady-state step is replayed from a captured
+ * hipGraph. */
+int ubp_step_n(ubp_engine *e, int nsteps, int use_graph);
+/* Current device dt (the dt the next ubp_step_n step will use); synchronous. */
+int ubp_get_dt(ubp_engine *e, float *dt);
+
+/* GpuKernel::step_profiled (backend/crates/kernel/src/gpu/mod.rs:1043-1278). */
+int ubp_step_profiled(ubp_engine *e, float dt, ubp_step_profile *out);
+
+/* SimulationKernel::particles (lib.rs:63; GPU readback gpu/buffers.rs:
+ * 360-477), into caller buffers in ORIGINAL particle order.  Synchronous. */
+int ubp_read_particles(ubp_engine *e, const ubp_particles_out *out);
+/* SimulationKernel::particle_count (lib.rs:69). */
+size_t ubp_particle_count(const ubp_engine *e);
+/* SimulationKernel::error_metrics (lib.rs:66, body lib.rs:341-377): FP32 max
+ * density variation, FP64 device reductions for energy and mass drift. */
+int ubp_error_metrics(ubp_engine *e, ubp_error_metrics_t *out);
+/* SimulationKernel::save_checkpoint / restore_checkpoint (lib.rs:71-77,
+ * CPU body lib.rs:383-395): device-to-device snapshot of the particle state
+ * and the neighbour structure.  Return 1 on success, 0 when there is nothing
+ * to restore, negative on error. */
+int ubp_save_checkpoint(ubp_engine *e);
+int ubp_restore_checkpoint(ubp_engine *e);
+
+/* Parity debug output for the current positions (original order):
+ * Morton cell key per particle, the (key, original index) sorted permutation
+ * (perm[k] = original index in sorted slot k), and accepted neighbour counts
+ * from the last density pass.  Replaces the reference grid state
+ * NeighborGrid::{cell_indices, sorted_indices} (neighbor.rs:11-23). */
+int ubp_debug_bins(ubp_engine *e, uint32_t *keys, uint32_t *perm, uint32_t *nbr_counts);
+
+/* Grid geometry of the handle: dims[3], Morton bits[3]. */
+int ubp_grid_info(const ubp_engine *e, uint32_t dims[3], uint32_t bits[3]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UBP_H */

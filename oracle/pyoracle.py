@@ -1,0 +1,311 @@
+"""ctypes front end of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` import this module, and only as the checker.  The product path
+(``ultraballpit_amd``) never imports it.
+
+The oracle restates the reference ``CpuKernel`` (backend/crates/kernel/src/
+lib.rs:91-396 and the modules it calls); see ``ubp_oracle.cpp`` for the
+per-function citations.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libubp_oracle.so")
+
+KAHAN = 1
+EOS_EXACT = 2
+DIFF_RHO_FLOOR = 4
+MORTON = 8
+ENGINE_CHECKPOINT = 16
+#: flags of the engine arithmetic the HIP backend reproduces bit for bit
+ENGINE = KAHAN | EOS_EXACT | ENGINE_CHECKPOINT
+#: the reference arithmetic (plain sums, powf EOS, reference checkpoint)
+FAITHFUL = 0
+
+FIELDS = ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az",
+          "density", "pressure", "mass", "temperature")
+
+
+class _Params(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int32),
+        ("flags", C.c_uint32),
+        ("h", C.c_float),
+        ("gravity", C.c_float * 3),
+        ("speed_of_sound", C.c_float),
+        ("cfl_number", C.c_float),
+        ("viscosity", C.c_float),
+        ("domain_min", C.c_float * 3),
+        ("domain_max", C.c_float * 3),
+        ("alpha", C.c_float),
+        ("beta", C.c_float),
+        ("diff_rho_floor", C.c_float),
+    ]
+
+
+_lib = None
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (g++, no GPU needed)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        build()
+    L = C.CDLL(LIB_PATH)
+    fp = C.POINTER(C.c_float)
+    u8p = C.POINTER(C.c_uint8)
+    u32p = C.POINTER(C.c_uint32)
+    dp = C.POINTER(C.c_double)
+    L.orc_create.restype = C.c_void_p
+    L.orc_create.argtypes = [C.POINTER(_Params), C.c_size_t] + [fp] * 12 + [u8p, C.c_size_t] + [fp] * 7
+    L.orc_destroy.argtypes = [C.c_void_p]
+    L.orc_set_sdf.argtypes = [C.c_void_p, fp, C.c_float, u32p, fp]
+    L.orc_set_threads.argtypes = [C.c_int]
+    L.orc_set_threads.restype = C.c_int
+    L.orc_step.argtypes = [C.c_void_p, C.c_double]
+    L.orc_timestep.argtypes = [C.c_void_p]
+    L.orc_timestep.restype = C.c_double
+    L.orc_step_auto.argtypes = [C.c_void_p, C.c_int, dp]
+    L.orc_get.argtypes = [C.c_void_p, C.c_int, dp]
+    L.orc_get_boundary_pressure.argtypes = [C.c_void_p, dp]
+    L.orc_error_metrics.argtypes = [C.c_void_p, dp]
+    L.orc_energy_mass.argtypes = [C.c_void_p, dp]
+    L.orc_bins.argtypes = [C.c_void_p, C.c_int, u32p, u32p, u32p]
+    L.orc_grid_dims.argtypes = [C.c_void_p, u32p]
+    L.orc_save_checkpoint.argtypes = [C.c_void_p]
+    L.orc_restore_checkpoint.argtypes = [C.c_void_p]
+    L.orc_wendland.argtypes = [C.c_int, C.c_double, C.c_double]
+    L.orc_wendland.restype = C.c_double
+    L.orc_wendland_grad.argtypes = [C.c_int] + [C.c_double] * 5 + [dp]
+    L.orc_tait.argtypes = [C.c_int, C.c_int] + [C.c_double] * 4
+    L.orc_tait.restype = C.c_double
+    L.orc_wall_omega.argtypes = [C.c_int, C.c_double]
+    L.orc_wall_omega.restype = C.c_double
+    L.orc_wall_gamma.argtypes = [C.c_int, C.c_double]
+    L.orc_wall_gamma.restype = C.c_double
+    L.orc_det_log.argtypes = [C.c_double]
+    L.orc_det_log.restype = C.c_double
+    L.orc_det_exp.argtypes = [C.c_double]
+    L.orc_det_exp.restype = C.c_double
+    L.orc_morton.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, u32p]
+    L.orc_morton.restype = C.c_uint32
+    L.orc_sdf_sample.argtypes = [fp, C.c_float, u32p, fp, fp, fp]
+    L.orc_sdf_gradient.argtypes = [fp, C.c_float, u32p, fp, fp, fp]
+    L.orc_neighbors.argtypes = [C.c_size_t, fp, fp, fp, C.c_float, fp, fp, C.c_float,
+                                C.c_size_t, u32p, C.c_size_t]
+    _lib = L
+    return L
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float)) if a is not None else None
+
+
+def _u32(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def set_threads(n: int) -> int:
+    return lib().orc_set_threads(int(n))
+
+
+class Oracle:
+    """A CPU simulation instance (FP32 or FP64 twin) on a scene dict.
+
+    ``scene`` holds numpy arrays ``x y z vx vy vz mass density temperature
+    fluid_type`` (original particle order), optional ``boundary`` arrays, the
+    constructor scalars of ``CpuKernel::new`` (lib.rs:138-148), the Monaghan
+    ``alpha``/``beta`` and an optional ``sdf`` grid.
+    """
+
+    def __init__(self, scene: dict, precision: int = 32, flags: int = ENGINE):
+        L = lib()
+        self.n = int(len(scene["x"]))
+        p = _Params()
+        p.precision = precision
+        p.flags = flags
+        p.h = scene["h"]
+        p.gravity[:] = list(scene["gravity"])
+        p.speed_of_sound = scene["speed_of_sound"]
+        p.cfl_number = scene["cfl_number"]
+        p.viscosity = scene.get("viscosity", 0.0)
+        p.domain_min[:] = list(scene["domain_min"])
+        p.domain_max[:] = list(scene["domain_max"])
+        p.alpha = scene.get("alpha", 1.0)
+        p.beta = scene.get("beta", 2.0)
+        p.diff_rho_floor = scene.get("diff_rho_floor", 1.0)
+        self._keep = []
+
+        def arr(k, d=None):
+            if k not in scene or scene[k] is None:
+                return None
+            a = _f32(scene[k])
+            self._keep.append(a)
+            return a
+
+        ft = np.ascontiguousarray(scene.get("fluid_type", np.zeros(self.n, np.uint8)), dtype=np.uint8)
+        self._keep.append(ft)
+        b = scene.get("boundary")
+        nb = 0 if b is None else int(len(b["x"]))
+        bargs = [None] * 7
+        if nb:
+            bargs = []
+            for k in ("x", "y", "z", "mass", "nx", "ny", "nz"):
+                a = _f32(b[k])
+                self._keep.append(a)
+                bargs.append(a)
+        self.h = L.orc_create(C.byref(p), self.n,
+                              *[_fp(arr(k)) for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay",
+                                                      "az", "mass", "density", "temperature")],
+                              ft.ctypes.data_as(C.POINTER(C.c_uint8)), nb,
+                              *[_fp(a) for a in bargs])
+        self.nb = nb
+        sdf = scene.get("sdf")
+        if sdf is not None:
+            o = _f32(sdf["origin"])
+            d = np.ascontiguousarray(sdf["dims"], dtype=np.uint32)
+            data = _f32(sdf["data"]).ravel()
+            self._keep += [o, d, data]
+            L.orc_set_sdf(self.h, _fp(o), float(sdf["cell"]), _u32(d), _fp(data))
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().orc_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def step(self, dt: float):
+        lib().orc_step(self.h, float(dt))
+
+    def timestep(self) -> float:
+        return lib().orc_timestep(self.h)
+
+    def step_auto(self, nsteps: int) -> np.ndarray:
+        dts = np.zeros(nsteps, np.float64)
+        lib().orc_step_auto(self.h, int(nsteps), _dp(dts))
+        return dts
+
+    def get(self, field: str) -> np.ndarray:
+        out = np.empty(self.n, np.float64)
+        lib().orc_get(self.h, FIELDS.index(field), _dp(out))
+        return out
+
+    def state(self) -> dict:
+        return {k: self.get(k) for k in FIELDS}
+
+    def boundary_pressure(self) -> np.ndarray:
+        out = np.empty(self.nb, np.float64)
+        lib().orc_get_boundary_pressure(self.h, _dp(out))
+        return out
+
+    def error_metrics(self) -> np.ndarray:
+        out = np.empty(3, np.float64)
+        lib().orc_error_metrics(self.h, _dp(out))
+        return out
+
+    def energy_mass(self) -> np.ndarray:
+        out = np.empty(4, np.float64)
+        lib().orc_energy_mass(self.h, _dp(out))
+        return out
+
+    def bins(self, morton: bool = True):
+        keys = np.empty(self.n, np.uint32)
+        perm = np.empty(self.n, np.uint32)
+        cnt = np.empty(self.n, np.uint32)
+        lib().orc_bins(self.h, 1 if morton else 0, _u32(keys), _u32(perm), _u32(cnt))
+        return keys, perm, cnt
+
+    def grid_dims(self):
+        d = np.empty(3, np.uint32)
+        lib().orc_grid_dims(self.h, _u32(d))
+        return d
+
+    def save_checkpoint(self) -> bool:
+        return bool(lib().orc_save_checkpoint(self.h))
+
+    def restore_checkpoint(self) -> bool:
+        return bool(lib().orc_restore_checkpoint(self.h))
+
+
+# ---- scalar helpers ----
+
+def wendland(r, h, precision=32):
+    return lib().orc_wendland(precision, float(r), float(h))
+
+
+def wendland_grad(dx, dy, dz, r, h, precision=32):
+    out = np.zeros(3, np.float64)
+    lib().orc_wendland_grad(precision, float(dx), float(dy), float(dz), float(r), float(h), _dp(out))
+    return out
+
+
+def tait(rho, rho0, cs, gamma, exact=False, precision=32):
+    return lib().orc_tait(precision, 1 if exact else 0, float(rho), float(rho0), float(cs), float(gamma))
+
+
+def wall_omega(s, precision=64):
+    return lib().orc_wall_omega(precision, float(s))
+
+
+def wall_gamma(s, precision=64):
+    return lib().orc_wall_gamma(precision, float(s))
+
+
+def det_log(x):
+    return lib().orc_det_log(float(x))
+
+
+def det_exp(x):
+    return lib().orc_det_exp(float(x))
+
+
+def morton(cx, cy, cz, bits):
+    b = np.ascontiguousarray(bits, dtype=np.uint32)
+    return int(lib().orc_morton(int(cx), int(cy), int(cz), _u32(b)))
+
+
+def sdf_sample(sdf: dict, p):
+    o = _f32(sdf["origin"]); d = np.ascontiguousarray(sdf["dims"], np.uint32)
+    data = _f32(sdf["data"]).ravel(); pt = _f32(p); out = np.zeros(1, np.float32)
+    ok = lib().orc_sdf_sample(_fp(o), float(sdf["cell"]), _u32(d), _fp(data), _fp(pt), _fp(out))
+    return float(out[0]), bool(ok)
+
+
+def sdf_gradient(sdf: dict, p):
+    o = _f32(sdf["origin"]); d = np.ascontiguousarray(sdf["dims"], np.uint32)
+    data = _f32(sdf["data"]).ravel(); pt = _f32(p); out = np.zeros(3, np.float32)
+    ok = lib().orc_sdf_gradient(_fp(o), float(sdf["cell"]), _u32(d), _fp(data), _fp(pt), _fp(out))
+    return out.astype(np.float64), bool(ok)
+
+
+def neighbors(x, y, z, cell, dmin, dmax, radius, i, cap=4096):
+    x = _f32(x); y = _f32(y); z = _f32(z)
+    mn = _f32(dmin); mx = _f32(dmax)
+    out = np.zeros(cap, np.uint32)
+    k = lib().orc_neighbors(len(x), _fp(x), _fp(y), _fp(z), float(cell), _fp(mn), _fp(mx),
+                            float(radius), int(i), _u32(out), cap)
+    return out[:min(k, cap)].astype(np.int64)

@@ -1,0 +1,1017 @@
+// ubp_oracle.cpp -- CPU ORACLE for the WCSPH step (TEST INFRASTRUCTURE ONLY).
+//
+// This file is the checker, never the product: only tests/, the smoke entry
+// point and the bench CPU-baseline leg load it.  The product path is the HIP
+// engine in ultraballpit_amd/csrc and never calls into this library.
+//
+// It restates the reference CPU backend (`CpuKernel`, Rust) in C++, templated
+// on the real type so every FP32 run has an FP64 twin on identical inputs.
+// Sources followed (paths relative to the reference repository):
+//   backend/crates/kernel/src/lib.rs:138-395     construction, compute_forces,
+//                                                step (KDK + bootstrap), metrics,
+//                                                checkpoint
+//   backend/crates/kernel/src/neighbor.rs:29-183 grid dims, pos_to_cell,
+//                                                stable counting sort, 27-cell walk
+//   backend/crates/kernel/src/sph.rs:35-628      Wendland C2, gradient, density,
+//                                                delta-SPH, XSPH, pressure,
+//                                                viscosity, gravity, c_s, CFL dt
+//   backend/crates/kernel/src/eos.rs:7-47        Tait EOS + constants
+//   backend/crates/kernel/src/boundary.rs:87-291 Adami pressures, repulsion,
+//                                                domain clamp
+//   backend/crates/orchestrator/src/geometry.rs:99-181  GridSDF sample/gradient
+//
+// Two arithmetic modes share one code path, selected by flags:
+//   faithful (flags = 0, alpha = 1, beta = 2): the reference's operation order,
+//       plain summation and powf EOS.
+//   engine (ORC_KAHAN | ORC_EOS_EXACT, runtime alpha/beta, optional SDF wall):
+//       the same per-particle operation order with compensated sums, the EOS
+//       power taken in FP64, and the SDF wall model.  The HIP engine is
+//       required to match this mode bit for bit.
+// Every function below notes the reference lines it follows.
+//
+// Parity pinning: see the oracle tests (closed forms, brute-force neighbour
+// enumeration, invariants, FP64 twin).  Nothing in this file can be checked
+// against the Rust binary itself (no Rust toolchain), which is the one link
+// that stays unpinned.
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+enum : uint32_t {
+  ORC_KAHAN = 1u,          // compensated density/diffusion/pressure/viscous sums
+  ORC_EOS_EXACT = 2u,      // (rho/rho0)^gamma - 1 evaluated in FP64
+  ORC_DIFF_RHO_FLOOR = 4u, // delta-SPH divides by max(rho_j, floor) (wgpu variant)
+  ORC_MORTON = 8u,         // debug keys are Morton codes (else linear hash)
+  ORC_ENGINE_CHECKPOINT = 16u, // restore also restores the neighbour grid
+};
+
+template <class R> static inline R pi_c() { return R(3.14159265358979323846); }
+
+// sph.rs:20  WENDLAND_C2_NORM_3D = 21 / (16 * PI), evaluated in R.
+template <class R> static inline R wnorm() {
+  const R sixteen_pi = R(16) * pi_c<R>();
+  return R(21) / sixteen_pi;
+}
+
+// sph.rs:35-46
+template <class R> static inline R wendland(R r, R h) {
+  R q = r / h;
+  if (q >= R(2)) return R(0);
+  R h3 = h * h * h;
+  R omq = R(1) - R(0.5) * q;
+  R t = omq * omq;
+  R t4 = t * t;
+  return wnorm<R>() / h3 * t4 * (R(1) + R(2) * q);
+}
+
+// sph.rs:68-85
+template <class R>
+static inline void wendland_grad(R dx, R dy, R dz, R r, R h, R& gx, R& gy, R& gz) {
+  R q = r / h;
+  if (q >= R(2) || r < R(1.0e-12)) { gx = gy = gz = R(0); return; }
+  R h3 = h * h * h;
+  R omq = R(1) - R(0.5) * q;
+  R t3 = omq * omq * omq;
+  R dw_dr = wnorm<R>() / (h3 * h) * (R(-5) * q) * t3;
+  R inv_r = R(1) / r;
+  gx = dw_dr * dx * inv_r;
+  gy = dw_dr * dy * inv_r;
+  gz = dw_dr * dz * inv_r;
+}
+
+// Neumaier-free classic Kahan accumulator; identical op order on every backend.
+template <class R> struct Acc {
+  R s, c;
+  bool kahan;
+  inline void init(R v, bool k) { s = v; c = R(0); kahan = k; }
+  inline void add(R v) {
+    if (!kahan) { s = s + v; return; }
+    R y = v - c;
+    R t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+};
+
+// Own FP64 exp/log so the FP64 EOS power is the same bits everywhere
+// (host libm and device libm differ).  log via atanh series after
+// mantissa/exponent split; exp via range reduction and a Taylor series.
+static inline double det_log(double x) {
+  int e = 0;
+  double m = std::frexp(x, &e);  // x = m * 2^e, m in [0.5,1)
+  if (m < 0.70710678118654752440) { m *= 2.0; e -= 1; }
+  double t = (m - 1.0) / (m + 1.0);
+  double t2 = t * t;
+  double term = t, sum = 0.0;
+  for (int k = 1; k < 60; k += 2) { sum += term / double(k); term *= t2; }
+  return 2.0 * sum + double(e) * 0.69314718055994530942;
+}
+static inline double det_exp(double x) {
+  // Cody-Waite: ln2 = LN2_HI + LN2_LO with kf * LN2_HI exact
+  const double LN2_HI = 6.93147180369123816490e-01, LN2_LO = 1.90821492927058770002e-10;
+  double kf = std::floor(x / 0.69314718055994530942 + 0.5);
+  double r = (x - kf * LN2_HI) - kf * LN2_LO;
+  double term = 1.0, sum = 1.0;
+  for (int k = 1; k < 30; ++k) { term = term * r / double(k); sum += term; }
+  return std::ldexp(sum, int(kf));
+}
+
+// eos.rs:39-47 (faithful) and the engine variant.
+template <class R> static inline R tait(R rho, R rho0, R cs, R gamma, bool exact) {
+  R b = rho0 * cs * cs / gamma;
+  R ratio = rho / rho0;
+  R pw1;
+  if (!exact) {
+    pw1 = std::pow(ratio, gamma) - R(1);
+  } else {
+    double t = double(ratio);
+    double tp;
+    if (gamma == R(7)) {
+      double t2 = t * t;
+      double t3 = t2 * t;
+      double t6 = t3 * t3;
+      tp = t6 * t;
+    } else {
+      tp = (t > 0.0) ? det_exp(double(gamma) * det_log(t)) : 0.0;
+    }
+    pw1 = R(tp - 1.0);
+  }
+  R p = b * pw1;
+  return (p > R(0)) ? p : R(0);
+}
+
+// Engine SDF wall: fraction of the Wendland C2 kernel volume that lies beyond
+// a planar wall at distance s*h, Omega(s) = (21/8) int_s^2 f(q) q (q-s) dq with
+// f(q) = (1-q/2)^4 (1+2q); Gamma(s) = -dOmega/ds.  In factored form
+// Omega = (1-s/2)^6 (1/2 + 3/4 s + 3/8 s^2), Gamma = (1-s/2)^5 (3/4 + 15/8 s
+// + 3/2 s^2): dyadic coefficients, and no cancellation as s -> 2.
+template <class R> static inline R wall_omega(R s) {
+  if (s >= R(2)) return R(0);
+  if (s < R(0)) s = R(0);
+  R t = R(1) - R(0.5) * s;
+  R t2 = t * t;
+  R t4 = t2 * t2;
+  R t6 = t4 * t2;
+  R p = (R(0.375) * s + R(0.75)) * s + R(0.5);
+  return t6 * p;
+}
+template <class R> static inline R wall_gamma(R s) {
+  if (s >= R(2)) return R(0);
+  if (s < R(0)) s = R(0);
+  R t = R(1) - R(0.5) * s;
+  R t2 = t * t;
+  R t4 = t2 * t2;
+  R t5 = t4 * t;
+  R p = (R(1.5) * s + R(1.875)) * s + R(0.75);
+  return t5 * p;
+}
+
+// Morton interleave of (cx, cy, cz) with per-axis bit widths; bit level l
+// contributes x, then y, then z (each only while the axis still has bits).
+static inline uint32_t morton3(uint32_t cx, uint32_t cy, uint32_t cz, const uint32_t bits[3]) {
+  uint32_t code = 0, pos = 0;
+  uint32_t mb = std::max(bits[0], std::max(bits[1], bits[2]));
+  for (uint32_t l = 0; l < mb; ++l) {
+    if (l < bits[0]) { code |= ((cx >> l) & 1u) << pos; ++pos; }
+    if (l < bits[1]) { code |= ((cy >> l) & 1u) << pos; ++pos; }
+    if (l < bits[2]) { code |= ((cz >> l) & 1u) << pos; ++pos; }
+  }
+  return code;
+}
+static inline uint32_t bits_for(uint32_t n) {
+  uint32_t b = 0;
+  while ((1u << b) < n) ++b;
+  return b;
+}
+
+template <class R> struct State {
+  std::vector<R> x, y, z, vx, vy, vz, ax, ay, az, rho, prs, m, temp;
+  std::vector<uint8_t> ft;
+  size_t size() const { return x.size(); }
+};
+
+// neighbor.rs:11-183
+template <class R> struct Grid {
+  R cell;
+  R gmin[3];
+  uint32_t dims[3];
+  std::vector<uint32_t> cell_idx, sorted, offsets, counts;
+
+  void init(R cs, const R dmin[3], const R dmax[3]) {
+    cell = cs;
+    for (int a = 0; a < 3; ++a) {
+      gmin[a] = dmin[a];
+      R d = std::ceil((dmax[a] - dmin[a]) / cs);
+      if (!(d >= R(1))) d = R(1);  // .max(1.0)
+      dims[a] = uint32_t(d);
+    }
+    size_t tc = size_t(dims[0]) * dims[1] * dims[2];
+    offsets.assign(tc, 0);
+    counts.assign(tc, 0);
+  }
+  size_t total() const { return size_t(dims[0]) * dims[1] * dims[2]; }
+  // neighbor.rs:57-71 -- floor().max(0).min(dims-1) as u32
+  inline uint32_t axis_cell(R p, int a) const {
+    R c = std::floor((p - gmin[a]) / cell);
+    if (!(c >= R(0))) c = R(0);
+    R hi = R(dims[a] - 1);
+    if (c > hi) c = hi;
+    return uint32_t(c);
+  }
+  inline void pos_to_cell(R px, R py, R pz, uint32_t& cx, uint32_t& cy, uint32_t& cz) const {
+    cx = axis_cell(px, 0); cy = axis_cell(py, 1); cz = axis_cell(pz, 2);
+  }
+  inline uint32_t hash(uint32_t cx, uint32_t cy, uint32_t cz) const {
+    return cx + cy * dims[0] + cz * dims[0] * dims[1];
+  }
+  // neighbor.rs:82-122: count, exclusive prefix, stable scatter in index order.
+  void update(const std::vector<R>& x, const std::vector<R>& y, const std::vector<R>& z) {
+    size_t n = x.size(), tc = total();
+    cell_idx.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+      uint32_t cx, cy, cz;
+      pos_to_cell(x[i], y[i], z[i], cx, cy, cz);
+      cell_idx[i] = hash(cx, cy, cz);
+    }
+    counts.assign(tc, 0);
+    for (size_t i = 0; i < n; ++i) counts[cell_idx[i]] += 1;
+    offsets.assign(tc, 0);
+    uint32_t run = 0;
+    for (size_t c = 0; c < tc; ++c) { offsets[c] = run; run += counts[c]; }
+    sorted.resize(n);
+    std::vector<uint32_t> heads(offsets);
+    for (size_t i = 0; i < n; ++i) sorted[heads[cell_idx[i]]++] = uint32_t(i);
+  }
+  // neighbor.rs:129-183 -- dz, dy, dx order; within a cell ascending index;
+  // skip j == i; accept dist^2 <= radius^2.
+  template <class F>
+  inline void for_each(size_t i, const std::vector<R>& x, const std::vector<R>& y,
+                       const std::vector<R>& z, R radius, F&& f) const {
+    R px = x[i], py = y[i], pz = z[i];
+    uint32_t cx, cy, cz;
+    pos_to_cell(px, py, pz, cx, cy, cz);
+    R rsq = radius * radius;
+    for (int dz = -1; dz <= 1; ++dz) {
+      int nz = int(cz) + dz;
+      if (nz < 0 || nz >= int(dims[2])) continue;
+      for (int dy = -1; dy <= 1; ++dy) {
+        int ny = int(cy) + dy;
+        if (ny < 0 || ny >= int(dims[1])) continue;
+        for (int dx = -1; dx <= 1; ++dx) {
+          int nx = int(cx) + dx;
+          if (nx < 0 || nx >= int(dims[0])) continue;
+          uint32_t c = hash(uint32_t(nx), uint32_t(ny), uint32_t(nz));
+          uint32_t st = offsets[c], cnt = counts[c];
+          for (uint32_t s = st; s < st + cnt; ++s) {
+            size_t j = sorted[s];
+            if (j == i) continue;
+            R ddx = px - x[j];
+            R ddy = py - y[j];
+            R ddz = pz - z[j];
+            R d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            if (d2 <= rsq) f(j);
+          }
+        }
+      }
+    }
+  }
+};
+
+// geometry.rs:11-20, 99-181 (GridSDF): trilinear sample, z fastest,
+// "outside" for nodes beyond the grid; gradient by central differences at
+// +-cell/2, normalised when |g| > 1e-8.
+template <class R> struct Sdf {
+  bool on = false;
+  R origin[3];
+  R cell;
+  uint32_t dims[3];
+  std::vector<float> data;
+
+  inline R node(uint32_t i, uint32_t j, uint32_t k) const {
+    uint32_t idx = i * dims[1] * dims[2] + j * dims[2] + k;
+    return (size_t(idx) < data.size()) ? R(data[idx]) : std::numeric_limits<R>::max();
+  }
+  // returns false when the point is outside the node grid (reference: f32::MAX)
+  inline bool sample(R px, R py, R pz, R& out) const {
+    R gx = (px - origin[0]) / cell;
+    R gy = (py - origin[1]) / cell;
+    R gz = (pz - origin[2]) / cell;
+    R fgx = std::floor(gx), fgy = std::floor(gy), fgz = std::floor(gz);
+    // `as i32` saturates and maps NaN to 0; only the bounds test matters here.
+    if (!(fgx >= R(0)) || !(fgy >= R(0)) || !(fgz >= R(0))) {
+      if (std::isnan(fgx) || std::isnan(fgy) || std::isnan(fgz)) return false;
+      return false;
+    }
+    if (fgx >= R(dims[0] - 1) || fgy >= R(dims[1] - 1) || fgz >= R(dims[2] - 1)) return false;
+    int i0 = int(fgx), j0 = int(fgy), k0 = int(fgz);
+    int i1 = i0 + 1, j1 = j0 + 1, k1 = k0 + 1;
+    R fx = gx - R(i0), fy = gy - R(j0), fz = gz - R(k0);
+    R c000 = node(i0, j0, k0), c001 = node(i0, j0, k1);
+    R c010 = node(i0, j1, k0), c011 = node(i0, j1, k1);
+    R c100 = node(i1, j0, k0), c101 = node(i1, j0, k1);
+    R c110 = node(i1, j1, k0), c111 = node(i1, j1, k1);
+    R c00 = c000 * (R(1) - fz) + c001 * fz;
+    R c01 = c010 * (R(1) - fz) + c011 * fz;
+    R c10 = c100 * (R(1) - fz) + c101 * fz;
+    R c11 = c110 * (R(1) - fz) + c111 * fz;
+    R c0 = c00 * (R(1) - fy) + c01 * fy;
+    R c1 = c10 * (R(1) - fy) + c11 * fy;
+    out = c0 * (R(1) - fx) + c1 * fx;
+    return true;
+  }
+  // false when any central-difference sample leaves the grid
+  inline bool gradient(R px, R py, R pz, R& nx, R& ny, R& nz) const {
+    R hh = cell * R(0.5);
+    R a, b;
+    R two_h = R(2) * hh;
+    if (!sample(px + hh, py, pz, a) || !sample(px - hh, py, pz, b)) return false;
+    R dx = (a - b) / two_h;
+    if (!sample(px, py + hh, pz, a) || !sample(px, py - hh, pz, b)) return false;
+    R dy = (a - b) / two_h;
+    if (!sample(px, py, pz + hh, a) || !sample(px, py, pz - hh, b)) return false;
+    R dz = (a - b) / two_h;
+    R mag = std::sqrt(dx * dx + dy * dy + dz * dz);
+    if (mag > R(1e-8)) { nx = dx / mag; ny = dy / mag; nz = dz / mag; }
+    else { nx = ny = nz = R(0); }
+    return true;
+  }
+  // wall distance + normal for the engine wall model; false = no wall here
+  inline bool wall(R px, R py, R pz, R& d, R& nx, R& ny, R& nz) const {
+    if (!on) return false;
+    if (!sample(px, py, pz, d)) return false;
+    return gradient(px, py, pz, nx, ny, nz);
+  }
+};
+
+struct Params {
+  int32_t precision;
+  uint32_t flags;
+  float h;
+  float gravity[3];
+  float speed_of_sound;
+  float cfl_number;
+  float viscosity;
+  float domain_min[3];
+  float domain_max[3];
+  float alpha;
+  float beta;
+  float diff_rho_floor;
+};
+
+template <class R> struct Sim {
+  Params prm;
+  State<R> p;
+  // boundary.rs:20-37
+  std::vector<R> bx, by, bz, bm, bnx, bny, bnz, bp;
+  Grid<R> grid;
+  Sdf<R> sdf;
+  R h, cs, g[3], dmin[3], dmax[3], alpha, beta, floor_rho;
+  bool kahan, eos_exact;
+  double e0 = 0, m0 = 0;
+  bool needs_init = true;
+  bool has_ckpt = false;
+  State<R> ckpt;
+  Grid<R> ckpt_grid;
+  std::vector<uint32_t> last_nbr_count;
+
+  static R rest_density(uint8_t ft) { return ft == 0 ? R(1000.0) : R(1.204); }  // eos.rs:7-10
+  static R gamma_of(uint8_t ft) { return ft == 0 ? R(7.0) : R(1.4); }            // eos.rs:13-19
+
+  // lib.rs:177-194
+  double energy() const {
+    double e = 0.0;
+    for (size_t i = 0; i < p.size(); ++i) {
+      double m = double(p.m[i]);
+      double vx = double(p.vx[i]), vy = double(p.vy[i]), vz = double(p.vz[i]);
+      e += 0.5 * m * (vx * vx + vy * vy + vz * vz);
+      double x = double(p.x[i]), y = double(p.y[i]), z = double(p.z[i]);
+      e -= m * (double(g[0]) * x + double(g[1]) * y + double(g[2]) * z);
+    }
+    return e;
+  }
+  double mass() const {
+    double s = 0.0;
+    for (size_t i = 0; i < p.size(); ++i) s += double(p.m[i]);
+    return s;
+  }
+
+  // lib.rs:203-283
+  void compute_forces(R dt) {
+    const size_t n = p.size();
+    const R sr = R(2) * h;
+    grid.update(p.x, p.y, p.z);
+
+    // sph.rs:169-216 -- delta-SPH from the OLD density
+    std::vector<R> diffusion(n);
+    const R eta_sq = R(0.01) * h * h;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long li = 0; li < long(n); ++li) {
+      size_t i = size_t(li);
+      Acc<R> acc;
+      acc.init(R(0), kahan);
+      grid.for_each(i, p.x, p.y, p.z, sr, [&](size_t j) {
+        R dx = p.x[i] - p.x[j];
+        R dy = p.y[i] - p.y[j];
+        R dz = p.z[i] - p.z[j];
+        R dist_sq = dx * dx + dy * dy + dz * dz;
+        R r = std::sqrt(dist_sq);
+        R gx, gy, gz;
+        wendland_grad(dx, dy, dz, r, h, gx, gy, gz);
+        R rho_j = p.rho[j];
+        if (floor_rho > R(0) && !(rho_j >= floor_rho)) rho_j = floor_rho;
+        R v_j = p.m[j] / rho_j;
+        R r_dot_grad = dx * gx + dy * gy + dz * gz;
+        R density_diff = p.rho[i] - p.rho[j];
+        acc.add(v_j * density_diff * R(2) * r_dot_grad / (dist_sq + eta_sq));
+      });
+      diffusion[i] = R(0.1) * h * cs * acc.s;
+    }
+
+    // sph.rs:112-160 -- summation density (+ boundary particles), then the
+    // engine wall term as the final summand.
+    std::vector<R> rho_new(n);
+    std::vector<uint32_t> nbrc(n);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long li = 0; li < long(n); ++li) {
+      size_t i = size_t(li);
+      Acc<R> acc;
+      acc.init(p.m[i] * wendland<R>(R(0), h), kahan);
+      uint32_t cnt = 0;
+      grid.for_each(i, p.x, p.y, p.z, sr, [&](size_t j) {
+        R dx = p.x[i] - p.x[j];
+        R dy = p.y[i] - p.y[j];
+        R dz = p.z[i] - p.z[j];
+        R r = std::sqrt(dx * dx + dy * dy + dz * dz);
+        acc.add(p.m[j] * wendland<R>(r, h));
+        ++cnt;
+      });
+      R px = p.x[i], py = p.y[i], pz = p.z[i];
+      for (size_t b = 0; b < bx.size(); ++b) {
+        R dx = px - bx[b];
+        R dy = py - by[b];
+        R dz = pz - bz[b];
+        R r = std::sqrt(dx * dx + dy * dy + dz * dz);
+        if (r < sr) acc.add(bm[b] * wendland<R>(r, h));
+      }
+      R d, wnx, wny, wnz;
+      if (sdf.wall(px, py, pz, d, wnx, wny, wnz)) {
+        R s = d / h;
+        if (s < R(2)) acc.add(rest_density(p.ft[i]) * wall_omega<R>(s));
+      }
+      rho_new[i] = acc.s;
+      nbrc[i] = cnt;
+    }
+    last_nbr_count.swap(nbrc);
+    // lib.rs:232-234
+    for (size_t i = 0; i < n; ++i) p.rho[i] = rho_new[i] + diffusion[i] * dt;
+
+    // sph.rs:295-312
+    for (size_t i = 0; i < n; ++i)
+      p.prs[i] = tait<R>(p.rho[i], rest_density(p.ft[i]), cs, gamma_of(p.ft[i]), eos_exact);
+
+    // boundary.rs:87-131 (brute force over all fluid particles, index order)
+    for (size_t b = 0; b < bx.size(); ++b) {
+      R wp = R(0), ws = R(0);
+      for (size_t f = 0; f < n; ++f) {
+        R dx = bx[b] - p.x[f];
+        R dy = by[b] - p.y[f];
+        R dz = bz[b] - p.z[f];
+        R r = std::sqrt(dx * dx + dy * dy + dz * dz);
+        if (r < sr) {
+          R w = wendland<R>(r, h);
+          R gdr = g[0] * dx + g[1] * dy + g[2] * dz;
+          R pe = p.prs[f] + p.rho[f] * gdr;
+          wp += w * pe;
+          ws += w;
+        }
+      }
+      if (ws > R(1.0e-12)) {
+        R v = wp / ws;
+        bp[b] = (v > R(0)) ? v : R(0);
+      } else {
+        bp[b] = R(0);
+      }
+    }
+
+    // lib.rs:248-252 zero; sph.rs:322-417 pressure; sph.rs:432-492 viscous;
+    // sph.rs:502-508 gravity; boundary.rs:143-181 repulsion; engine wall.
+    const R r0 = R(0.5) * h;
+    const R d_rep = R(10.0) * R(9.81) * R(0.01);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long li = 0; li < long(n); ++li) {
+      size_t i = size_t(li);
+      R rho_i = p.rho[i];
+      R pi_over = p.prs[i] / (rho_i * rho_i);
+      Acc<R> fx, fy, fz;
+      fx.init(R(0), kahan); fy.init(R(0), kahan); fz.init(R(0), kahan);
+      Acc<R> vxs, vys, vzs;
+      vxs.init(R(0), kahan); vys.init(R(0), kahan); vzs.init(R(0), kahan);
+      grid.for_each(i, p.x, p.y, p.z, sr, [&](size_t j) {
+        R dx = p.x[i] - p.x[j];
+        R dy = p.y[i] - p.y[j];
+        R dz = p.z[i] - p.z[j];
+        R r = std::sqrt(dx * dx + dy * dy + dz * dz);
+        R gx, gy, gz;
+        wendland_grad(dx, dy, dz, r, h, gx, gy, gz);
+        // pressure (sph.rs:352-368)
+        R pj_over = p.prs[j] / (p.rho[j] * p.rho[j]);
+        R factor = -p.m[i] * p.m[j] * (pi_over + pj_over);
+        fx.add(factor * gx); fy.add(factor * gy); fz.add(factor * gz);
+        // viscous (sph.rs:456-481)
+        R dvx = p.vx[i] - p.vx[j];
+        R dvy = p.vy[i] - p.vy[j];
+        R dvz = p.vz[i] - p.vz[j];
+        R vr = dvx * dx + dvy * dy + dvz * dz;
+        if (vr < R(0)) {
+          R r_sq = r * r;
+          R mu = h * vr / (r_sq + eta_sq);
+          R rho_avg = R(0.5) * (p.rho[i] + p.rho[j]);
+          R pij = (-alpha * cs * mu + beta * mu * mu) / rho_avg;
+          R vf = -p.m[j] * pij;
+          vxs.add(vf * gx); vys.add(vf * gy); vzs.add(vf * gz);
+        }
+      });
+      // boundary pressure contributions (sph.rs:376-408)
+      if (!bx.empty()) {
+        R px = p.x[i], py = p.y[i], pz = p.z[i];
+        R m_i = p.m[i];
+        R pic = (p.prs[i] > R(0)) ? p.prs[i] : R(0);
+        R pic_over = pic / (rho_i * rho_i);
+        R brho = rest_density(p.ft[i]);
+        for (size_t b = 0; b < bx.size(); ++b) {
+          R dx = px - bx[b];
+          R dy = py - by[b];
+          R dz = pz - bz[b];
+          R r = std::sqrt(dx * dx + dy * dy + dz * dz);
+          if (r < sr) {
+            R gx, gy, gz;
+            wendland_grad(dx, dy, dz, r, h, gx, gy, gz);
+            R pb_over = bp[b] / (brho * brho);
+            R factor = -m_i * bm[b] * (pic_over + pb_over);
+            fx.add(factor * gx); fy.add(factor * gy); fz.add(factor * gz);
+          }
+        }
+      }
+      R ax = R(0), ay = R(0), az = R(0);
+      ax += fx.s / p.m[i];
+      ay += fy.s / p.m[i];
+      az += fz.s / p.m[i];
+      ax += vxs.s; ay += vys.s; az += vzs.s;
+      ax += g[0]; ay += g[1]; az += g[2];
+      // boundary.rs:156-180
+      if (!bx.empty()) {
+        R px = p.x[i], py = p.y[i], pz = p.z[i];
+        for (size_t b = 0; b < bx.size(); ++b) {
+          R dx = px - bx[b];
+          R dy = py - by[b];
+          R dz = pz - bz[b];
+          R r = std::sqrt(dx * dx + dy * dy + dz * dz);
+          if (r < r0 && r > R(1.0e-12)) {
+            R s = R(1) - r / r0;
+            R fm = d_rep * s * s / r0;
+            R inv_r = R(1) / r;
+            ax += fm * dx * inv_r;
+            ay += fm * dy * inv_r;
+            az += fm * dz * inv_r;
+          }
+        }
+      }
+      // engine SDF wall: semi-analytic wall pressure + reference repulsion profile
+      R d, wnx, wny, wnz;
+      if (sdf.wall(p.x[i], p.y[i], p.z[i], d, wnx, wny, wnz)) {
+        R s = d / h;
+        if (s < R(2)) {
+          R pic = (p.prs[i] > R(0)) ? p.prs[i] : R(0);
+          R coef = R(2) * (pic / (rho_i * rho_i)) * rest_density(p.ft[i]) * wall_gamma<R>(s) / h;
+          ax += coef * wnx; ay += coef * wny; az += coef * wnz;
+          if (d < r0) {
+            R dd = (d > R(0)) ? d : R(0);
+            R sr0 = R(1) - dd / r0;
+            R fm = d_rep * sr0 * sr0 / r0;
+            ax += fm * wnx; ay += fm * wny; az += fm * wnz;
+          }
+        }
+      }
+      p.ax[i] = ax; p.ay[i] = ay; p.az[i] = az;
+    }
+  }
+
+  // lib.rs:287-335
+  void step(R dt) {
+    const size_t n = p.size();
+    const R half_dt = R(0.5) * dt;
+    if (needs_init) { compute_forces(dt); needs_init = false; }
+    for (size_t i = 0; i < n; ++i) {
+      p.vx[i] += p.ax[i] * half_dt;
+      p.vy[i] += p.ay[i] * half_dt;
+      p.vz[i] += p.az[i] * half_dt;
+    }
+    // sph.rs:236-280 (XSPH) on the grid from the last compute_forces
+    std::vector<R> dvx(n), dvy(n), dvz(n);
+    const R sr = R(2) * h;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long li = 0; li < long(n); ++li) {
+      size_t i = size_t(li);
+      R cx = R(0), cy = R(0), cz = R(0);
+      grid.for_each(i, p.x, p.y, p.z, sr, [&](size_t j) {
+        R dx = p.x[i] - p.x[j];
+        R dy = p.y[i] - p.y[j];
+        R dz = p.z[i] - p.z[j];
+        R r = std::sqrt(dx * dx + dy * dy + dz * dz);
+        R w = wendland<R>(r, h);
+        R rho_avg = R(0.5) * (p.rho[i] + p.rho[j]);
+        R den = (rho_avg >= R(1)) ? rho_avg : R(1);  // f32::max(1.0)
+        R factor = p.m[j] / den * w;
+        cx += factor * (p.vx[j] - p.vx[i]);
+        cy += factor * (p.vy[j] - p.vy[i]);
+        cz += factor * (p.vz[j] - p.vz[i]);
+      });
+      dvx[i] = R(0.5) * cx; dvy[i] = R(0.5) * cy; dvz[i] = R(0.5) * cz;
+    }
+    for (size_t i = 0; i < n; ++i) {
+      p.x[i] += (p.vx[i] + dvx[i]) * dt;
+      p.y[i] += (p.vy[i] + dvy[i]) * dt;
+      p.z[i] += (p.vz[i] + dvz[i]) * dt;
+    }
+    // engine SDF wall projection (d < 0: back to d = 0, restitution 0.2)
+    if (sdf.on) {
+      for (size_t i = 0; i < n; ++i) {
+        R d, wnx, wny, wnz;
+        if (!sdf.wall(p.x[i], p.y[i], p.z[i], d, wnx, wny, wnz)) continue;
+        if (d < R(0)) {
+          p.x[i] -= d * wnx; p.y[i] -= d * wny; p.z[i] -= d * wnz;
+          R vn = p.vx[i] * wnx + p.vy[i] * wny + p.vz[i] * wnz;
+          if (vn < R(0)) {
+            R k = (R(1) + R(0.2)) * vn;
+            p.vx[i] -= k * wnx; p.vy[i] -= k * wny; p.vz[i] -= k * wnz;
+          }
+        }
+      }
+    }
+    // boundary.rs:239-291
+    const R rest = R(0.2);
+    for (size_t i = 0; i < n; ++i) {
+      if (p.x[i] < dmin[0]) { p.x[i] = dmin[0]; if (p.vx[i] < R(0)) p.vx[i] = -rest * p.vx[i]; }
+      if (p.x[i] > dmax[0]) { p.x[i] = dmax[0]; if (p.vx[i] > R(0)) p.vx[i] = -rest * p.vx[i]; }
+      if (p.y[i] < dmin[1]) { p.y[i] = dmin[1]; if (p.vy[i] < R(0)) p.vy[i] = -rest * p.vy[i]; }
+      if (p.y[i] > dmax[1]) { p.y[i] = dmax[1]; if (p.vy[i] > R(0)) p.vy[i] = -rest * p.vy[i]; }
+      if (p.z[i] < dmin[2]) { p.z[i] = dmin[2]; if (p.vz[i] < R(0)) p.vz[i] = -rest * p.vz[i]; }
+      if (p.z[i] > dmax[2]) { p.z[i] = dmax[2]; if (p.vz[i] > R(0)) p.vz[i] = -rest * p.vz[i]; }
+    }
+    compute_forces(dt);
+    for (size_t i = 0; i < n; ++i) {
+      p.vx[i] += p.ax[i] * half_dt;
+      p.vy[i] += p.ay[i] * half_dt;
+      p.vz[i] += p.az[i] * half_dt;
+    }
+  }
+
+  // sph.rs:591-628
+  R timestep() const {
+    R max_signal = cs, max_acc = R(0);
+    for (size_t i = 0; i < p.size(); ++i) {
+      R v = std::sqrt(p.vx[i] * p.vx[i] + p.vy[i] * p.vy[i] + p.vz[i] * p.vz[i]);
+      R sig = v + cs;
+      if (sig > max_signal) max_signal = sig;
+      R a = std::sqrt(p.ax[i] * p.ax[i] + p.ay[i] * p.ay[i] + p.az[i] * p.az[i]);
+      if (a > max_acc) max_acc = a;
+    }
+    R cfl = R(prm.cfl_number);
+    R dt_cfl = cfl * h / max_signal;
+    R dt_force = (max_acc > R(1.0e-12)) ? R(0.25) * std::sqrt(h / max_acc) : R(0.01);
+    R dt = (dt_force < dt_cfl) ? dt_force : dt_cfl;  // f32::min (no NaN here)
+    if (dt < R(1.0e-8)) dt = R(1.0e-8);
+    if (dt > R(0.01)) dt = R(0.01);
+    return dt;
+  }
+
+  // lib.rs:341-377
+  void metrics(double out[3]) const {
+    float maxv = 0.0f;
+    for (size_t i = 0; i < p.size(); ++i) {
+      R r0 = rest_density(p.ft[i]);
+      float v = float(std::fabs(p.rho[i] - r0) / r0);
+      if (v > maxv) maxv = v;
+    }
+    double e = energy();
+    float ed = (std::fabs(e0) > 1.0e-12) ? float(std::fabs((e - e0) / e0)) : float(std::fabs(e - e0));
+    double m = mass();
+    float md = (std::fabs(m0) > 1.0e-12) ? float(std::fabs((m - m0) / m0)) : float(std::fabs(m - m0));
+    out[0] = maxv; out[1] = ed; out[2] = md;
+  }
+};
+
+struct Handle {
+  int prec;
+  Sim<float>* f = nullptr;
+  Sim<double>* d = nullptr;
+  std::string err;
+};
+
+template <class R>
+static Sim<R>* make_sim(const Params& prm, size_t n, const float* x, const float* y, const float* z,
+                        const float* vx, const float* vy, const float* vz, const float* ax,
+                        const float* ay, const float* az, const float* m,
+                        const float* rho, const float* temp, const uint8_t* ft, size_t nb,
+                        const float* bx, const float* by, const float* bz, const float* bm,
+                        const float* bnx, const float* bny, const float* bnz) {
+  Sim<R>* s = new Sim<R>();
+  s->prm = prm;
+  auto cp = [n](std::vector<R>& dst, const float* src, float dflt) {
+    dst.resize(n);
+    for (size_t i = 0; i < n; ++i) dst[i] = src ? R(src[i]) : R(dflt);
+  };
+  State<R>& p = s->p;
+  cp(p.x, x, 0); cp(p.y, y, 0); cp(p.z, z, 0);
+  cp(p.vx, vx, 0); cp(p.vy, vy, 0); cp(p.vz, vz, 0);
+  cp(p.ax, ax, 0); cp(p.ay, ay, 0); cp(p.az, az, 0);
+  cp(p.rho, rho, 1000.0f); cp(p.m, m, 0); cp(p.temp, temp, 293.15f);
+  p.prs.assign(n, R(0));
+  p.ft.resize(n);
+  for (size_t i = 0; i < n; ++i) p.ft[i] = ft ? ft[i] : 0;
+  auto cpb = [nb](std::vector<R>& dst, const float* src) {
+    dst.resize(nb);
+    for (size_t i = 0; i < nb; ++i) dst[i] = R(src[i]);
+  };
+  if (nb) {
+    cpb(s->bx, bx); cpb(s->by, by); cpb(s->bz, bz); cpb(s->bm, bm);
+    cpb(s->bnx, bnx); cpb(s->bny, bny); cpb(s->bnz, bnz);
+    s->bp.assign(nb, R(0));
+  }
+  s->h = R(prm.h);
+  s->cs = R(prm.speed_of_sound);
+  for (int a = 0; a < 3; ++a) {
+    s->g[a] = R(prm.gravity[a]);
+    s->dmin[a] = R(prm.domain_min[a]);
+    s->dmax[a] = R(prm.domain_max[a]);
+  }
+  s->alpha = R(prm.alpha);
+  s->beta = R(prm.beta);
+  s->kahan = (prm.flags & ORC_KAHAN) != 0;
+  s->eos_exact = (prm.flags & ORC_EOS_EXACT) != 0;
+  s->floor_rho = (prm.flags & ORC_DIFF_RHO_FLOOR) ? R(prm.diff_rho_floor) : R(0);
+  s->grid.init(R(2) * s->h, s->dmin, s->dmax);  // lib.rs:149-150
+  s->m0 = s->mass();
+  s->e0 = s->energy();
+  return s;
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+struct orc_params {
+  int32_t precision;  // 32 or 64
+  uint32_t flags;
+  float h;
+  float gravity[3];
+  float speed_of_sound;
+  float cfl_number;
+  float viscosity;
+  float domain_min[3];
+  float domain_max[3];
+  float alpha;
+  float beta;
+  float diff_rho_floor;
+};
+
+void* orc_create(const orc_params* prm, size_t n, const float* x, const float* y, const float* z,
+                 const float* vx, const float* vy, const float* vz, const float* ax,
+                 const float* ay, const float* az, const float* mass,
+                 const float* density, const float* temperature, const uint8_t* fluid_type,
+                 size_t nb, const float* bx, const float* by, const float* bz, const float* bmass,
+                 const float* bnx, const float* bny, const float* bnz) {
+  Params P;
+  std::memcpy(&P, prm, sizeof(P));
+  Handle* hd = new Handle();
+  hd->prec = P.precision;
+  if (P.precision == 64)
+    hd->d = make_sim<double>(P, n, x, y, z, vx, vy, vz, ax, ay, az, mass, density, temperature, fluid_type, nb,
+                             bx, by, bz, bmass, bnx, bny, bnz);
+  else
+    hd->f = make_sim<float>(P, n, x, y, z, vx, vy, vz, ax, ay, az, mass, density, temperature, fluid_type, nb,
+                            bx, by, bz, bmass, bnx, bny, bnz);
+  return hd;
+}
+
+void orc_destroy(void* h) {
+  Handle* hd = static_cast<Handle*>(h);
+  delete hd->f;
+  delete hd->d;
+  delete hd;
+}
+
+int orc_set_sdf(void* h, const float origin[3], float cell, const uint32_t dims[3], const float* data) {
+  Handle* hd = static_cast<Handle*>(h);
+  size_t nn = size_t(dims[0]) * dims[1] * dims[2];
+  auto set = [&](auto* s) {
+    s->sdf.on = true;
+    for (int a = 0; a < 3; ++a) { s->sdf.origin[a] = origin[a]; s->sdf.dims[a] = dims[a]; }
+    s->sdf.cell = cell;
+    s->sdf.data.assign(data, data + nn);
+  };
+  if (hd->d) set(hd->d); else set(hd->f);
+  return 0;
+}
+
+int orc_set_threads(int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+  return omp_get_max_threads();
+#else
+  (void)nthreads;
+  return 1;
+#endif
+}
+
+void orc_step(void* h, double dt) {
+  Handle* hd = static_cast<Handle*>(h);
+  if (hd->d) hd->d->step(dt); else hd->f->step(float(dt));
+}
+
+double orc_timestep(void* h) {
+  Handle* hd = static_cast<Handle*>(h);
+  return hd->d ? hd->d->timestep() : double(hd->f->timestep());
+}
+
+// step with dt = compute_timestep(state) before every step (the orchestrator
+// runner loop, runner.rs:208-238, without the optimistic 1.5 dt branch)
+void orc_step_auto(void* h, int nsteps, double* dts) {
+  Handle* hd = static_cast<Handle*>(h);
+  for (int k = 0; k < nsteps; ++k) {
+    double dt = orc_timestep(h);
+    if (dts) dts[k] = dt;
+    orc_step(h, dt);
+  }
+}
+
+// field ids: 0 x 1 y 2 z 3 vx 4 vy 5 vz 6 ax 7 ay 8 az 9 rho 10 p 11 m 12 T
+void orc_get(void* h, int field, double* out) {
+  Handle* hd = static_cast<Handle*>(h);
+  auto get = [&](auto* s) {
+    auto& p = s->p;
+    const auto* src = &p.x;
+    switch (field) {
+      case 0: src = &p.x; break;   case 1: src = &p.y; break;   case 2: src = &p.z; break;
+      case 3: src = &p.vx; break;  case 4: src = &p.vy; break;  case 5: src = &p.vz; break;
+      case 6: src = &p.ax; break;  case 7: src = &p.ay; break;  case 8: src = &p.az; break;
+      case 9: src = &p.rho; break; case 10: src = &p.prs; break; case 11: src = &p.m; break;
+      default: src = &p.temp; break;
+    }
+    for (size_t i = 0; i < src->size(); ++i) out[i] = double((*src)[i]);
+  };
+  if (hd->d) get(hd->d); else get(hd->f);
+}
+
+void orc_get_boundary_pressure(void* h, double* out) {
+  Handle* hd = static_cast<Handle*>(h);
+  auto get = [&](auto* s) { for (size_t i = 0; i < s->bp.size(); ++i) out[i] = double(s->bp[i]); };
+  if (hd->d) get(hd->d); else get(hd->f);
+}
+
+void orc_error_metrics(void* h, double out[3]) {
+  Handle* hd = static_cast<Handle*>(h);
+  if (hd->d) hd->d->metrics(out); else hd->f->metrics(out);
+}
+
+void orc_energy_mass(void* h, double out[4]) {
+  Handle* hd = static_cast<Handle*>(h);
+  auto get = [&](auto* s) { out[0] = s->energy(); out[1] = s->mass(); out[2] = s->e0; out[3] = s->m0; };
+  if (hd->d) get(hd->d); else get(hd->f);
+}
+
+// Binning debug output for the CURRENT positions: Morton (or linear) cell key
+// per particle (original order), the stable (key, original index) sorted
+// permutation, and accepted neighbour counts from the last compute_forces.
+int orc_bins(void* h, int morton, uint32_t* keys, uint32_t* perm, uint32_t* nbr_counts) {
+  Handle* hd = static_cast<Handle*>(h);
+  auto run = [&](auto* s) {
+    auto& g = s->grid;
+    size_t n = s->p.size();
+    uint32_t bits[3] = {bits_for(g.dims[0]), bits_for(g.dims[1]), bits_for(g.dims[2])};
+    std::vector<uint32_t> k(n);
+    for (size_t i = 0; i < n; ++i) {
+      uint32_t cx, cy, cz;
+      g.pos_to_cell(s->p.x[i], s->p.y[i], s->p.z[i], cx, cy, cz);
+      k[i] = morton ? morton3(cx, cy, cz, bits) : g.hash(cx, cy, cz);
+    }
+    if (keys) std::copy(k.begin(), k.end(), keys);
+    if (perm) {
+      std::vector<uint32_t> idx(n);
+      for (size_t i = 0; i < n; ++i) idx[i] = uint32_t(i);
+      std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return k[a] < k[b]; });
+      std::copy(idx.begin(), idx.end(), perm);
+    }
+    if (nbr_counts) {
+      if (s->last_nbr_count.size() == n)
+        std::copy(s->last_nbr_count.begin(), s->last_nbr_count.end(), nbr_counts);
+      else
+        std::fill(nbr_counts, nbr_counts + n, 0u);
+    }
+  };
+  if (hd->d) run(hd->d); else run(hd->f);
+  return 0;
+}
+
+void orc_grid_dims(void* h, uint32_t dims[3]) {
+  Handle* hd = static_cast<Handle*>(h);
+  if (hd->d) { for (int a = 0; a < 3; ++a) dims[a] = hd->d->grid.dims[a]; }
+  else { for (int a = 0; a < 3; ++a) dims[a] = hd->f->grid.dims[a]; }
+}
+
+// lib.rs:383-395; ORC_ENGINE_CHECKPOINT additionally snapshots the grid
+int orc_save_checkpoint(void* h) {
+  Handle* hd = static_cast<Handle*>(h);
+  auto run = [&](auto* s) { s->ckpt = s->p; s->ckpt_grid = s->grid; s->has_ckpt = true; };
+  if (hd->d) run(hd->d); else run(hd->f);
+  return 1;
+}
+int orc_restore_checkpoint(void* h) {
+  Handle* hd = static_cast<Handle*>(h);
+  int ok = 0;
+  auto run = [&](auto* s) {
+    if (!s->has_ckpt) return;
+    s->p = s->ckpt;
+    if (s->prm.flags & ORC_ENGINE_CHECKPOINT) s->grid = s->ckpt_grid;
+    s->has_ckpt = false;
+    ok = 1;
+  };
+  if (hd->d) run(hd->d); else run(hd->f);
+  return ok;
+}
+
+// ---- scalar entry points for closed-form tests ----
+double orc_wendland(int prec, double r, double h) {
+  return prec == 64 ? wendland<double>(r, h) : double(wendland<float>(float(r), float(h)));
+}
+void orc_wendland_grad(int prec, double dx, double dy, double dz, double r, double h, double out[3]) {
+  if (prec == 64) { wendland_grad<double>(dx, dy, dz, r, h, out[0], out[1], out[2]); return; }
+  float a, b, c;
+  wendland_grad<float>(float(dx), float(dy), float(dz), float(r), float(h), a, b, c);
+  out[0] = a; out[1] = b; out[2] = c;
+}
+double orc_tait(int prec, int exact, double rho, double rho0, double cs, double gamma) {
+  if (prec == 64) return tait<double>(rho, rho0, cs, gamma, exact != 0);
+  return double(tait<float>(float(rho), float(rho0), float(cs), float(gamma), exact != 0));
+}
+double orc_wall_omega(int prec, double s) {
+  return prec == 64 ? wall_omega<double>(s) : double(wall_omega<float>(float(s)));
+}
+double orc_wall_gamma(int prec, double s) {
+  return prec == 64 ? wall_gamma<double>(s) : double(wall_gamma<float>(float(s)));
+}
+double orc_det_log(double x) { return det_log(x); }
+double orc_det_exp(double x) { return det_exp(x); }
+uint32_t orc_morton(uint32_t cx, uint32_t cy, uint32_t cz, const uint32_t bits[3]) {
+  return morton3(cx, cy, cz, bits);
+}
+// GridSDF::sample / gradient on a stand-alone grid (FP32 semantics)
+int orc_sdf_sample(const float origin[3], float cell, const uint32_t dims[3], const float* data,
+                   const float p[3], float* out) {
+  Sdf<float> s;
+  s.on = true;
+  for (int a = 0; a < 3; ++a) { s.origin[a] = origin[a]; s.dims[a] = dims[a]; }
+  s.cell = cell;
+  s.data.assign(data, data + size_t(dims[0]) * dims[1] * dims[2]);
+  float v;
+  if (!s.sample(p[0], p[1], p[2], v)) { *out = FLT_MAX; return 0; }
+  *out = v;
+  return 1;
+}
+int orc_sdf_gradient(const float origin[3], float cell, const uint32_t dims[3], const float* data,
+                     const float p[3], float out[3]) {
+  Sdf<float> s;
+  s.on = true;
+  for (int a = 0; a < 3; ++a) { s.origin[a] = origin[a]; s.dims[a] = dims[a]; }
+  s.cell = cell;
+  s.data.assign(data, data + size_t(dims[0]) * dims[1] * dims[2]);
+  return s.gradient(p[0], p[1], p[2], out[0], out[1], out[2]) ? 1 : 0;
+}
+// neighbour enumeration on a stand-alone point set, reference order
+int orc_neighbors(size_t n, const float* x, const float* y, const float* z, float cell,
+                  const float dmin[3], const float dmax[3], float radius, size_t i,
+                  uint32_t* out, size_t cap) {
+  Grid<float> g;
+  float mn[3] = {dmin[0], dmin[1], dmin[2]}, mx[3] = {dmax[0], dmax[1], dmax[2]};
+  g.init(cell, mn, mx);
+  std::vector<float> X(x, x + n), Y(y, y + n), Z(z, z + n);
+  g.update(X, Y, Z);
+  size_t k = 0;
+  g.for_each(i, X, Y, Z, radius, [&](size_t j) { if (k < cap) out[k] = uint32_t(j); ++k; });
+  return int(k);
+}
+
+}  // extern "C"

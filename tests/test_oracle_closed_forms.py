@@ -1,0 +1,185 @@
+"""Pin the oracle's scalar functions against mathematics, not against a
+retyped formula: normalisation integrals, derivatives by finite differences,
+textbook special values, and the reference's own unit tests
+(backend/crates/kernel/src/sph.rs:630-723, eos.rs:66-124)."""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+
+
+def gauss_legendre(f, a, b, n=64):
+    x, w = np.polynomial.legendre.leggauss(n)
+    t = 0.5 * (b - a) * x + 0.5 * (b + a)
+    return 0.5 * (b - a) * sum(wi * f(ti) for ti, wi in zip(t, w))
+
+
+# ---- Wendland C2 (sph.rs:35-85) ----
+
+@pytest.mark.parametrize("h", [0.1, 0.05, 4.0123e-3])
+def test_wendland_normalisation_radial(h):
+    # 4 pi int_0^{2h} W r^2 dr = 1 (FP64 twin, pins the 21/(16 pi) constant)
+    val = gauss_legendre(lambda r: 4 * math.pi * r * r * po.wendland(r, h, 64), 0.0, 2 * h)
+    assert abs(val - 1.0) < 1e-12
+
+
+def test_wendland_reference_unit_tests():
+    h = 0.1
+    # kernel_at_zero_distance (sph.rs:634-641)
+    assert abs(po.wendland(0.0, h) - 21 / (16 * math.pi) / h ** 3) < 1e-4
+    # kernel_at_support_radius / beyond (sph.rs:643-655)
+    assert abs(po.wendland(2 * h, h)) < 1e-10
+    assert po.wendland(0.5, 0.1) == 0.0
+    # positive inside (sph.rs:657-665)
+    for i in range(1, 20):
+        assert po.wendland(i * 0.01, h) > 0.0
+
+
+def test_wendland_riemann_3d_reference():
+    # kernel_normalization_numerical (sph.rs:696-722), coarser 40^3 midpoint sum
+    h = 0.1
+    n = 40
+    he = 2 * h
+    cell = 2 * he / n
+    c = -he + (np.arange(n) + 0.5) * cell
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    r = np.sqrt(X * X + Y * Y + Z * Z).ravel()
+    radii, inv = np.unique(np.round(r, 12), return_inverse=True)
+    w = np.array([po.wendland(float(x), h) for x in radii])
+    assert abs(w[inv].sum() * cell ** 3 - 1.0) < 0.02
+
+
+def test_wendland_c2_smoothness_at_support():
+    # C2 kernel: W, W' and W'' vanish at q = 2 (central differences, FP64)
+    h = 1.0
+    e = 1e-4
+    r = 2.0 - 2 * e
+    w0, w1, w2 = (po.wendland(r + k * e, h, 64) for k in (-1, 0, 1))
+    assert abs(w1) < 1e-10
+    assert abs((w2 - w0) / (2 * e)) < 1e-6
+
+
+@pytest.mark.parametrize("q", [0.05, 0.3, 0.9, 1.4, 1.95])
+def test_gradient_is_derivative_of_w(q):
+    # grad W = dW/dr * r_hat; dW/dr by central differences of W (FP64)
+    h = 0.02
+    d = np.array([0.3, -0.5, 0.81])
+    d = d / np.linalg.norm(d) * q * h
+    r = float(np.linalg.norm(d))
+    g = po.wendland_grad(*d, r, h, 64)
+    e = 1e-7 * h
+    dwdr = (po.wendland(r + e, h, 64) - po.wendland(r - e, h, 64)) / (2 * e)
+    assert np.allclose(g, dwdr * d / r, rtol=1e-6, atol=1e-9 / h ** 4)
+
+
+def test_gradient_reference_unit_tests():
+    assert list(po.wendland_grad(0, 0, 0, 0, 0.1)) == [0, 0, 0]       # sph.rs:667-673
+    assert list(po.wendland_grad(0.5, 0, 0, 0.5, 0.1)) == [0, 0, 0]   # sph.rs:675-681
+    g = po.wendland_grad(0.1, 0, 0, 0.1, 0.1)                        # sph.rs:683-694
+    assert g[0] < 0 and abs(g[1]) < 1e-10 and abs(g[2]) < 1e-10
+
+
+# ---- Tait EOS (eos.rs:39-47) ----
+
+def test_tait_reference_unit_tests():
+    for exact in (False, True):
+        assert abs(po.tait(1000.0, 1000.0, 20.0, 7.0, exact)) < 1e-3       # eos.rs:70-77
+        assert po.tait(1010.0, 1000.0, 20.0, 7.0, exact) > 0.0            # eos.rs:79-84
+        assert po.tait(990.0, 1000.0, 20.0, 7.0, exact) == 0.0            # eos.rs:86-94
+
+
+@pytest.mark.parametrize("rho", [1000.0001, 1000.37, 1003.0, 1012.5, 1100.0, 1.25, 1.5])
+def test_tait_exact_against_rational(rho):
+    # engine EOS: (ratio^gamma - 1) evaluated in FP64 then rounded once; the
+    # float result must equal B * fl32((ratio)^7 - 1) with the power taken in
+    # exact rational arithmetic (gamma = 7) -- or be within 1 ulp of it
+    rho0, cs, gamma = (1000.0, 30.0, 7.0) if rho > 10 else (1.204, 30.0, 7.0)
+    f = np.float32
+    b = f(f(f(rho0) * f(cs)) * f(cs)) / f(gamma)
+    ratio = f(f(rho) / f(rho0))
+    exact = Fraction(float(ratio)) ** 7 - 1
+    want = f(b) * f(float(exact))
+    want = want if want > 0 else f(0)
+    got = f(po.tait(rho, rho0, cs, gamma, exact=True))
+    assert abs(int(got.view(np.int32)) - int(np.float32(want).view(np.int32))) <= 1
+
+
+def test_tait_exact_beats_powf_near_rest():
+    # near rest the cancellation in (ratio^7 - 1) amplifies powf's rounding;
+    # the FP64-power variant must be at least as close to the FP64 twin
+    # reference: B * (ratio^7 - 1) in exact rationals for the SAME float ratio,
+    # so input rounding (shared by both variants) does not enter
+    f = np.float32
+    b = float(f(f(f(1000.0) * f(30.0)) * f(30.0)) / f(7.0))
+    worse = 0
+    e_sum = [0.0, 0.0]
+    for rho in np.linspace(1000.01, 1010.0, 101):
+        ratio = f(f(rho) / f(1000.0))
+        ref = b * float(Fraction(float(ratio)) ** 7 - 1)
+        e_exact = abs(po.tait(rho, 1000.0, 30.0, 7.0, True) - ref)
+        e_pow = abs(po.tait(rho, 1000.0, 30.0, 7.0, False) - ref)
+        e_sum[0] += e_exact
+        e_sum[1] += e_pow
+        # two roundings (power-minus-one to FP32, then the product): <= 2 ulp
+        worse += e_exact > 2.0 * float(np.spacing(f(ref)))
+    assert worse == 0
+    assert e_sum[0] < 0.2 * e_sum[1]
+
+
+@pytest.mark.parametrize("x", [1e-300, 0.01, 0.5, 0.999999, 1.0, 1.204, 3.0, 1e5, 7e200])
+def test_det_log(x):
+    assert abs(po.det_log(x) - math.log(x)) <= 2e-15 * max(1.0, abs(math.log(x)))
+
+
+@pytest.mark.parametrize("x", [-700.0, -3.3, -1e-9, 0.0, 0.4, 1.0, 12.5, 300.0])
+def test_det_exp(x):
+    assert abs(po.det_exp(x) - math.exp(x)) <= 4e-15 * math.exp(x)
+
+
+def test_tait_air_gamma_14():
+    # general-gamma path (det_exp(gamma * det_log(t))) against math.pow
+    for rho in (1.204 * 1.01, 1.204 * 1.3, 1.5):
+        f = np.float32
+        ratio = float(f(f(rho) / f(1.204)))
+        b = f(f(f(1.204) * f(20.0)) * f(20.0)) / f(1.4)
+        want = f(b) * f(ratio ** float(f(1.4)) - 1.0)
+        got = f(po.tait(rho, 1.204, 20.0, 1.4, exact=True))
+        assert abs(int(got.view(np.int32)) - int(np.float32(want).view(np.int32))) <= 1
+
+
+# ---- SDF wall integrals ----
+
+def test_wall_omega_special_values():
+    assert po.wall_omega(0.0) == 0.5
+    assert po.wall_omega(2.0) == 0.0
+    assert po.wall_omega(3.0) == 0.0
+    assert po.wall_omega(-1.0) == 0.5
+    assert po.wall_gamma(2.0) == 0.0
+
+
+@pytest.mark.parametrize("s", [0.0, 0.2, 0.75, 1.3, 1.9])
+def test_wall_omega_is_half_space_kernel_integral(s):
+    # Omega(d) = int_{z > d} W dV = 2 pi int_d^{2h} W(r) r (r - d) dr
+    h = 1.0
+    val = gauss_legendre(lambda r: 2 * math.pi * po.wendland(r, h, 64) * r * (r - s * h), s * h, 2 * h)
+    assert abs(val - po.wall_omega(s)) < 1e-12
+
+
+@pytest.mark.parametrize("s", [0.1, 0.6, 1.0, 1.7])
+def test_wall_gamma_is_minus_derivative(s):
+    e = 1e-6
+    d = (po.wall_omega(s + e) - po.wall_omega(s - e)) / (2 * e)
+    assert abs(-d - po.wall_gamma(s)) < 1e-8
+
+
+def test_wall_polynomials_fp32_equal_fp64_at_dyadic_points():
+    # coefficients are dyadic: at s = k/64 the FP32 Horner chain is exact
+    # only where no rounding occurs; require agreement to 1 ulp everywhere
+    for k in range(0, 129):
+        s = k / 64
+        a = np.float32(po.wall_omega(s, 32))
+        b = np.float32(po.wall_omega(s, 64))
+        assert abs(float(a) - float(b)) <= 2 * np.spacing(np.float32(max(abs(b), 1e-30)))

@@ -1,0 +1,395 @@
+"""ultraballpit_amd -- MI355X-native WCSPH step engine.
+
+Host-side mirror of the reference ``kernel`` crate's backend slot (paths
+relative to the reference repository):
+
+* ``HipKernel``      -- the ``SimulationKernel`` trait (backend/crates/kernel/
+                        src/lib.rs:58-78) plus the ``GpuKernel`` extras
+                        (gpu/mod.rs:996-1278), over the C ABI in include/ubp.h;
+* ``gpu_available``  -- gpu/mod.rs:171-182;
+* ``BackendType`` / ``create_kernel`` -- backend selection of
+                        orchestrator/src/config.rs:9-22 and
+                        orchestrator/src/lib.rs:142-266;
+* ``auto_tune_speed_of_sound`` / ``compute_timestep`` -- sph.rs:532-628.
+
+The compute path is the HIP library ``libubp_hip.so`` built from
+``csrc/``; nothing here falls back to a CPU implementation: a missing or
+unloadable library raises ``GpuInitError``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libubp_hip.so")
+
+FIELDS = ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az",
+          "density", "pressure", "mass", "temperature")
+
+
+class GpuInitError(RuntimeError):
+    """Engine creation failed (gpu/mod.rs:158-168)."""
+
+
+class EngineError(RuntimeError):
+    """A step or query on a live engine failed."""
+
+
+@dataclass
+class ErrorMetrics:
+    """lib.rs:38-47"""
+    max_density_variation: float
+    energy_conservation: float
+    mass_conservation: float
+
+
+class _ParticlesIn(C.Structure):
+    _fields_ = [("n", C.c_size_t)] + [(k, C.POINTER(C.c_float)) for k in
+                                       ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az",
+                                        "density", "mass", "temperature")] + \
+               [("fluid_type", C.POINTER(C.c_uint8))]
+
+
+class _BoundaryIn(C.Structure):
+    _fields_ = [("n", C.c_size_t)] + [(k, C.POINTER(C.c_float)) for k in
+                                       ("x", "y", "z", "mass", "nx", "ny", "nz")]
+
+
+class _ParticlesOut(C.Structure):
+    _fields_ = [(k, C.POINTER(C.c_float)) for k in FIELDS] + [("fluid_type", C.POINTER(C.c_uint8))]
+
+
+class _Metrics(C.Structure):
+    _fields_ = [("max_density_variation", C.c_float), ("energy_conservation", C.c_float),
+                ("mass_conservation", C.c_float)]
+
+
+class _Profile(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("grid_build_us", "density_us", "boundary_pressure_us",
+                                          "forces_us", "integrate_us", "readback_us", "total_us")]
+
+
+_lib = None
+
+#: every symbol include/ubp.h declares
+EXPORTS = ("ubp_create", "ubp_destroy", "ubp_last_error", "ubp_gpu_available",
+           "ubp_set_viscosity_coeffs", "ubp_set_sdf", "ubp_set_tuning", "ubp_step",
+           "ubp_step_async", "ubp_sync", "ubp_step_n", "ubp_get_dt", "ubp_step_profiled",
+           "ubp_read_particles", "ubp_particle_count", "ubp_error_metrics",
+           "ubp_save_checkpoint", "ubp_restore_checkpoint", "ubp_debug_bins", "ubp_grid_info")
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def lib():
+    """Load libubp_hip.so (raises GpuInitError when it is absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GpuInitError(f"HIP engine library not built: {LIB_PATH} (run __graft_entry__.build())")
+    try:
+        L = C.CDLL(LIB_PATH)
+    except OSError as e:
+        raise GpuInitError(f"cannot load {LIB_PATH}: {e}") from e
+    fp = C.POINTER(C.c_float)
+    u32p = C.POINTER(C.c_uint32)
+    L.ubp_create.restype = C.c_void_p
+    L.ubp_create.argtypes = [C.POINTER(_ParticlesIn), C.POINTER(_BoundaryIn), C.c_float, fp, C.c_float,
+                             C.c_float, C.c_float, fp, fp]
+    L.ubp_destroy.argtypes = [C.c_void_p]
+    L.ubp_last_error.restype = C.c_char_p
+    L.ubp_gpu_available.restype = C.c_int
+    L.ubp_set_viscosity_coeffs.argtypes = [C.c_void_p, C.c_float, C.c_float]
+    L.ubp_set_sdf.argtypes = [C.c_void_p, fp, C.c_float, u32p, fp]
+    L.ubp_set_tuning.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32]
+    L.ubp_step.argtypes = [C.c_void_p, C.c_float]
+    L.ubp_step_async.argtypes = [C.c_void_p, C.c_float]
+    L.ubp_sync.argtypes = [C.c_void_p]
+    L.ubp_step_n.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.ubp_get_dt.argtypes = [C.c_void_p, fp]
+    L.ubp_step_profiled.argtypes = [C.c_void_p, C.c_float, C.POINTER(_Profile)]
+    L.ubp_read_particles.argtypes = [C.c_void_p, C.POINTER(_ParticlesOut)]
+    L.ubp_particle_count.argtypes = [C.c_void_p]
+    L.ubp_particle_count.restype = C.c_size_t
+    L.ubp_error_metrics.argtypes = [C.c_void_p, C.POINTER(_Metrics)]
+    L.ubp_save_checkpoint.argtypes = [C.c_void_p]
+    L.ubp_restore_checkpoint.argtypes = [C.c_void_p]
+    L.ubp_debug_bins.argtypes = [C.c_void_p, u32p, u32p, u32p]
+    L.ubp_grid_info.argtypes = [C.c_void_p, u32p, u32p]
+    for name in EXPORTS:
+        getattr(L, name)
+    _lib = L
+    return L
+
+
+def _err() -> str:
+    return lib().ubp_last_error().decode(errors="replace")
+
+
+def _check(rc: int, what: str) -> int:
+    if rc < 0:
+        raise EngineError(f"{what} failed ({rc}): {_err()}")
+    return rc
+
+
+def gpu_available() -> bool:
+    """gpu/mod.rs:171-182 -- a gfx950 device is visible and the library loads."""
+    try:
+        return bool(lib().ubp_gpu_available())
+    except GpuInitError:
+        return False
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _u32p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+class HipKernel:
+    """``SimulationKernel`` on the MI355X engine.
+
+    Constructor arguments are those of ``GpuKernel::new`` / ``CpuKernel::new``
+    (gpu/mod.rs:190-200, lib.rs:138-148): a ``ParticleArrays``-shaped dict of
+    arrays (original particle order), a ``BoundaryParticles``-shaped dict or
+    None, h, gravity, speed of sound, CFL number, viscosity (ignored as in the
+    reference), domain bounds.
+    """
+
+    def __init__(self, particles: dict, boundary, h, gravity, speed_of_sound, cfl_number,
+                 viscosity, domain_min, domain_max):
+        L = lib()
+        n = int(len(particles["x"]))
+        self._n = n
+        keep = []
+
+        def arr(k):
+            v = particles.get(k)
+            if v is None:
+                return None
+            a = _f32(v)
+            if a.shape != (n,):
+                raise GpuInitError(f"particle array {k} has shape {a.shape}, expected ({n},)")
+            keep.append(a)
+            return _fp(a)
+
+        pin = _ParticlesIn()
+        pin.n = n
+        for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "density", "mass", "temperature"):
+            setattr(pin, k, arr(k))
+        ft = particles.get("fluid_type")
+        if ft is not None:
+            fta = np.ascontiguousarray(ft, dtype=np.uint8)
+            keep.append(fta)
+            pin.fluid_type = fta.ctypes.data_as(C.POINTER(C.c_uint8))
+        bin_ = None
+        if boundary is not None and len(boundary["x"]) > 0:
+            bin_ = _BoundaryIn()
+            bin_.n = len(boundary["x"])
+            for k in ("x", "y", "z", "mass", "nx", "ny", "nz"):
+                a = _f32(boundary[k])
+                keep.append(a)
+                setattr(bin_, k, _fp(a))
+        g = _f32(gravity)
+        dmin = _f32(domain_min)
+        dmax = _f32(domain_max)
+        self._h = L.ubp_create(C.byref(pin), C.byref(bin_) if bin_ is not None else None,
+                               float(h), _fp(g), float(speed_of_sound), float(cfl_number),
+                               float(viscosity), _fp(dmin), _fp(dmax))
+        if not self._h:
+            raise GpuInitError(_err())
+
+    # -- construction helpers --------------------------------------------
+    @classmethod
+    def from_scene(cls, scene: dict, boundary=None) -> "HipKernel":
+        k = cls(scene, boundary if boundary is not None else scene.get("boundary"), scene["h"],
+                scene["gravity"], scene["speed_of_sound"], scene["cfl_number"],
+                scene.get("viscosity", 0.0), scene["domain_min"], scene["domain_max"])
+        if "alpha" in scene or "beta" in scene:
+            k.set_viscosity_coeffs(scene.get("alpha", 1.0), scene.get("beta", 2.0))
+        if scene.get("sdf") is not None:
+            s = scene["sdf"]
+            k.set_sdf(s["origin"], s["cell"], s["dims"], s["data"])
+        return k
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ubp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- extension setters --------------------------------------------------
+    def set_viscosity_coeffs(self, alpha: float, beta: float):
+        _check(lib().ubp_set_viscosity_coeffs(self._h, float(alpha), float(beta)), "set_viscosity_coeffs")
+
+    def set_sdf(self, origin, cell, dims, data):
+        o = _f32(origin)
+        d = np.ascontiguousarray(dims, dtype=np.uint32)
+        v = _f32(data).ravel()
+        if v.size != int(np.prod(d.astype(np.int64))):
+            raise EngineError("SDF data size does not match dims")
+        _check(lib().ubp_set_sdf(self._h, _fp(o), float(cell), _u32p(d), _fp(v)), "set_sdf")
+
+    def set_tuning(self, block_size: int = 0, nbr_capacity: int = 0):
+        _check(lib().ubp_set_tuning(self._h, int(block_size), int(nbr_capacity)), "set_tuning")
+
+    def set_workgroup_size(self, wg: int):
+        """gpu/mod.rs:711-767 (32/64/128/256; 32 maps to 64, the CDNA wavefront)."""
+        if wg not in (32, 64, 128, 256):
+            raise ValueError("workgroup_size must be 32, 64, 128, or 256")
+        self.set_tuning(max(wg, 64), 0)
+
+    # -- SimulationKernel ----------------------------------------------------
+    def step(self, dt: float):
+        _check(lib().ubp_step(self._h, float(dt)), "step")
+
+    def step_no_sync(self, dt: float):
+        _check(lib().ubp_step_async(self._h, float(dt)), "step_no_sync")
+
+    def sync(self):
+        _check(lib().ubp_sync(self._h), "sync")
+
+    def step_n(self, nsteps: int, use_graph: bool = True):
+        """n steps with the CFL dt computed on the device before each step."""
+        _check(lib().ubp_step_n(self._h, int(nsteps), 1 if use_graph else 0), "step_n")
+
+    def dt(self) -> float:
+        v = np.zeros(1, np.float32)
+        _check(lib().ubp_get_dt(self._h, _fp(v)), "get_dt")
+        return float(v[0])
+
+    def particles(self) -> dict:
+        n = self._n
+        out = {k: np.empty(n, np.float32) for k in FIELDS}
+        out["fluid_type"] = np.empty(n, np.uint8)
+        po = _ParticlesOut()
+        for k in FIELDS:
+            setattr(po, k, _fp(out[k]))
+        po.fluid_type = out["fluid_type"].ctypes.data_as(C.POINTER(C.c_uint8))
+        _check(lib().ubp_read_particles(self._h, C.byref(po)), "read_particles")
+        return out
+
+    def error_metrics(self) -> ErrorMetrics:
+        m = _Metrics()
+        _check(lib().ubp_error_metrics(self._h, C.byref(m)), "error_metrics")
+        return ErrorMetrics(m.max_density_variation, m.energy_conservation, m.mass_conservation)
+
+    def particle_count(self) -> int:
+        return int(lib().ubp_particle_count(self._h))
+
+    def save_checkpoint(self) -> bool:
+        return _check(lib().ubp_save_checkpoint(self._h), "save_checkpoint") == 1
+
+    def restore_checkpoint(self) -> bool:
+        return _check(lib().ubp_restore_checkpoint(self._h), "restore_checkpoint") == 1
+
+    def step_profiled(self, dt: float) -> dict:
+        p = _Profile()
+        _check(lib().ubp_step_profiled(self._h, float(dt), C.byref(p)), "step_profiled")
+        return {k: int(getattr(p, k)) for k, _ in _Profile._fields_}
+
+    def debug_bins(self):
+        n = self._n
+        keys = np.empty(n, np.uint32)
+        perm = np.empty(n, np.uint32)
+        cnt = np.empty(n, np.uint32)
+        _check(lib().ubp_debug_bins(self._h, _u32p(keys), _u32p(perm), _u32p(cnt)), "debug_bins")
+        return keys, perm, cnt
+
+    def grid_info(self):
+        d = np.zeros(3, np.uint32)
+        b = np.zeros(3, np.uint32)
+        _check(lib().ubp_grid_info(self._h, _u32p(d), _u32p(b)), "grid_info")
+        return d, b
+
+
+# ---- host-side helpers of the reference API ------------------------------
+
+def auto_tune_speed_of_sound(gravity, domain_min, domain_max, c_s_config) -> float:
+    """sph.rs:532-572 in FP32."""
+    f = np.float32
+    g = [f(v) for v in gravity]
+    g_mag = np.sqrt(f(g[0] * g[0] + g[1] * g[1]) + g[2] * g[2])
+    if g_mag < f(1.0e-6):
+        return float(max(f(c_s_config), f(1.0)))
+    gn = [g[0] / g_mag, g[1] / g_mag, g[2] / g_mag]
+    ext = [f(domain_max[a]) - f(domain_min[a]) for a in range(3)]
+    height = abs(f(f(ext[0] * abs(gn[0]) + ext[1] * abs(gn[1])) + ext[2] * abs(gn[2])))
+    v_max = np.sqrt(f(f(2.0) * g_mag) * height)
+    c_auto = max(f(f(10.0) * v_max), f(1.0))
+    return float(min(c_auto, f(c_s_config)))
+
+
+def compute_timestep(particles: dict, h, speed_of_sound, cfl_number) -> float:
+    """sph.rs:591-628 in FP32 (max reductions are order independent)."""
+    f = np.float32
+    vx, vy, vz = (np.asarray(particles[k], np.float32) for k in ("vx", "vy", "vz"))
+    ax, ay, az = (np.asarray(particles[k], np.float32) for k in ("ax", "ay", "az"))
+    v = np.sqrt((vx * vx + vy * vy) + vz * vz)
+    sig = v + f(speed_of_sound)
+    max_signal = max(f(speed_of_sound), sig.max(initial=f(0)))
+    a = np.sqrt((ax * ax + ay * ay) + az * az)
+    max_acc = a.max(initial=f(0))
+    dt_cfl = f(f(cfl_number) * f(h)) / max_signal
+    dt_force = f(0.25) * np.sqrt(f(h) / max_acc) if max_acc > f(1.0e-12) else f(0.01)
+    dt = min(dt_cfl, dt_force)
+    return float(min(max(dt, f(1.0e-8)), f(0.01)))
+
+
+class BackendType(enum.Enum):
+    """orchestrator/src/config.rs:9-22"""
+    CPU = "cpu"
+    GPU = "gpu"
+    AUTO = "auto"
+
+
+def create_kernel(backend: BackendType, particles, boundary, h, gravity, speed_of_sound,
+                  cfl_number, viscosity, domain_min, domain_max, cpu_factory=None):
+    """orchestrator/src/lib.rs:142-266: auto-tune c_s, then build the backend.
+
+    ``GPU`` raises on init failure (the reference panics); ``AUTO`` falls back
+    to ``cpu_factory`` (the caller's CPU backend: the reference ``CpuKernel``
+    in a Rust integration) when the HIP engine cannot be created; ``CPU``
+    always uses ``cpu_factory``.
+    """
+    cs = auto_tune_speed_of_sound(gravity, domain_min, domain_max, speed_of_sound)
+    args = (particles, boundary, h, gravity, cs, cfl_number, viscosity, domain_min, domain_max)
+    if backend == BackendType.CPU:
+        if cpu_factory is None:
+            raise GpuInitError("CPU backend requested but no CPU factory was given")
+        return cpu_factory(*args)
+    if backend == BackendType.GPU:
+        return HipKernel(*args)
+    try:
+        if not gpu_available():
+            raise GpuInitError("no gfx950 device")
+        return HipKernel(*args)
+    except GpuInitError:
+        if cpu_factory is None:
+            raise
+        return cpu_factory(*args)

@@ -1,0 +1,58 @@
+"""Build the gfx950 engine library (libubp_hip.so) in-tree with hipcc.
+
+Floating-point flags are part of the engine's contract:
+  -ffp-contract=off                        no FMA contraction; the written
+                                           operation order is the executed one
+  -fhip-fp32-correctly-rounded-divide-sqrt IEEE-correct f32 '/' and sqrtf
+  -fno-gpu-flush-denormals-to-zero         keep FP32 denormals (as the host)
+  -target-feature -packed-fp32-ops         no packed FP32 (v_pk_*_f32) codegen
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libubp_hip.so")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
+    "-fno-gpu-flush-denormals-to-zero",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+
+def hipcc() -> str:
+    p = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(p):
+        raise RuntimeError("hipcc not found")
+    return p
+
+
+def build_engine(out: str = LIB, extra=None, verbose: bool = False) -> str:
+    src = os.path.join(CSRC, "ubp_engine.hip")
+    tmp = out + ".tmp"
+    cmd = [hipcc()] + HIP_FLAGS + list(extra or []) + ["-I", os.path.join(ROOT, "include"), "-o", tmp, src]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"hipcc failed ({r.returncode}) building {src}")
+    os.replace(tmp, out)
+    return out
+
+
+def build_all(verbose: bool = False) -> list[str]:
+    return [build_engine(verbose=verbose)]
+
+
+if __name__ == "__main__":
+    for p in build_all(verbose=True):
+        print("built", p)

@@ -1,0 +1,226 @@
+// ubp_device.hpp -- per-particle / per-pair arithmetic of the WCSPH step for
+// gfx950.  Every function reproduces the operation order of the reference CPU
+// backend (paths relative to the reference repository) so that, compiled with
+// -ffp-contract=off and IEEE-correct f32 divide/sqrt, the device computes the
+// same bits as the host for the same inputs.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ubp {
+
+// Engine constants, computed once on the host (in FP32, with the same
+// expressions the reference evaluates per call) and passed by value.
+struct DevParams {
+  uint32_t n;          // particles
+  uint32_t K;          // neighbour-list capacity per particle
+  float h;
+  float cs;            // speed of sound
+  float g[3];          // gravity
+  float dmin[3], dmax[3];
+  float cell;          // grid cell = 2h (kernel/src/lib.rs:149)
+  uint32_t dims[3];    // grid dims (neighbor.rs:31-35)
+  uint32_t ncodes;     // Morton code space = 2^(bx+by+bz)
+  float rsq;           // (2h)^2, neighbor.rs:144
+  float eta_sq;        // 0.01*h*h, sph.rs:177/442
+  float norm_h3;       // WENDLAND_C2_NORM_3D / h^3   (= W(0,h))
+  float norm_h4;       // WENDLAND_C2_NORM_3D / (h^3 h)
+  float diff_coef;     // DELTA_SPH * h * c_s, sph.rs:212
+  float nac;           // (-alpha) * c_s, sph.rs:472
+  float beta;
+  float cfl;
+  float r0;            // 0.5 h, boundary.rs:150
+  float d_rep;         // 10 * 9.81 * 0.01, boundary.rs:154
+  float diff_floor;    // 0 = CPU delta-SPH; >0 = max(rho_j, floor) (density.wgsl:186)
+  float b_tait[2];     // rho0 c^2 / gamma per fluid type (eos.rs:40)
+  float rho0[2];       // 1000, 1.204 (eos.rs:7-10)
+  float gamma[2];      // 7, 1.4 (eos.rs:13-19)
+  int sdf_on;
+  float so[3];         // SDF origin
+  float scell;         // SDF cell
+  uint32_t sd[3];      // SDF node dims
+  const float* sdf;    // SDF node values
+  const uint32_t* spread;  // Morton spread tables: x [dims0], y [dims1], z [dims2]
+};
+
+// ---- Wendland C2 (sph.rs:35-46, 68-85) ----
+__device__ __forceinline__ float wendland_q(float q, const DevParams& P) {
+  if (q >= 2.0f) return 0.0f;
+  float omq = 1.0f - 0.5f * q;
+  float t = omq * omq;
+  float t4 = t * t;
+  return P.norm_h3 * t4 * (1.0f + 2.0f * q);
+}
+__device__ __forceinline__ void wgrad_q(float dx, float dy, float dz, float r, float q,
+                                        const DevParams& P, float& gx, float& gy, float& gz) {
+  if (q >= 2.0f || r < 1.0e-12f) { gx = 0.0f; gy = 0.0f; gz = 0.0f; return; }
+  float omq = 1.0f - 0.5f * q;
+  float t3 = omq * omq * omq;
+  float dw = P.norm_h4 * (-5.0f * q) * t3;
+  float inv_r = 1.0f / r;
+  gx = dw * dx * inv_r;
+  gy = dw * dy * inv_r;
+  gz = dw * dz * inv_r;
+}
+
+// Compensated (Kahan) accumulator: t = s + y; c = (t - s) - y.
+struct Kahan {
+  float s, c;
+  __device__ __forceinline__ void init(float v) { s = v; c = 0.0f; }
+  __device__ __forceinline__ void add(float v) {
+    float y = v - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+};
+
+// ---- FP64 power for the Tait EOS (engine arithmetic, eos.rs:39-47) ----
+__device__ __forceinline__ double det_log(double x) {
+  int e = 0;
+  double m = frexp(x, &e);
+  if (m < 0.70710678118654752440) { m *= 2.0; e -= 1; }
+  double t = (m - 1.0) / (m + 1.0);
+  double t2 = t * t;
+  double term = t, sum = 0.0;
+  for (int k = 1; k < 60; k += 2) { sum += term / double(k); term *= t2; }
+  return 2.0 * sum + double(e) * 0.69314718055994530942;
+}
+__device__ __forceinline__ double det_exp(double x) {
+  const double LN2_HI = 6.93147180369123816490e-01, LN2_LO = 1.90821492927058770002e-10;
+  double kf = floor(x / 0.69314718055994530942 + 0.5);
+  double r = (x - kf * LN2_HI) - kf * LN2_LO;
+  double term = 1.0, sum = 1.0;
+  for (int k = 1; k < 30; ++k) { term = term * r / double(k); sum += term; }
+  return ldexp(sum, int(kf));
+}
+__device__ __forceinline__ float tait(float rho, int ft, const DevParams& P) {
+  float ratio = rho / P.rho0[ft];
+  double t = double(ratio);
+  double tp;
+  if (P.gamma[ft] == 7.0f) {
+    double t2 = t * t;
+    double t3 = t2 * t;
+    double t6 = t3 * t3;
+    tp = t6 * t;
+  } else {
+    tp = (t > 0.0) ? det_exp(double(P.gamma[ft]) * det_log(t)) : 0.0;
+  }
+  float pw1 = float(tp - 1.0);
+  float p = P.b_tait[ft] * pw1;
+  return (p > 0.0f) ? p : 0.0f;
+}
+
+// ---- uniform grid (neighbor.rs:57-77) + Morton code ----
+__device__ __forceinline__ uint32_t axis_cell(float p, float gmin, float cell, uint32_t dim) {
+  float c = floorf((p - gmin) / cell);
+  if (!(c >= 0.0f)) c = 0.0f;
+  float hi = float(dim - 1u);
+  if (c > hi) c = hi;
+  return uint32_t(c);
+}
+__device__ __forceinline__ void pos_to_cell(const DevParams& P, float x, float y, float z,
+                                            uint32_t& cx, uint32_t& cy, uint32_t& cz) {
+  cx = axis_cell(x, P.dmin[0], P.cell, P.dims[0]);
+  cy = axis_cell(y, P.dmin[1], P.cell, P.dims[1]);
+  cz = axis_cell(z, P.dmin[2], P.cell, P.dims[2]);
+}
+__device__ __forceinline__ uint32_t morton(const DevParams& P, uint32_t cx, uint32_t cy, uint32_t cz) {
+  return P.spread[cx] | P.spread[P.dims[0] + cy] | P.spread[P.dims[0] + P.dims[1] + cz];
+}
+
+// ---- GridSDF (orchestrator/src/geometry.rs:99-175) ----
+__device__ __forceinline__ float sdf_node(const DevParams& P, uint32_t i, uint32_t j, uint32_t k) {
+  return P.sdf[i * P.sd[1] * P.sd[2] + j * P.sd[2] + k];
+}
+__device__ __forceinline__ bool sdf_sample(const DevParams& P, float px, float py, float pz, float& out) {
+  float gx = (px - P.so[0]) / P.scell;
+  float gy = (py - P.so[1]) / P.scell;
+  float gz = (pz - P.so[2]) / P.scell;
+  float fgx = floorf(gx), fgy = floorf(gy), fgz = floorf(gz);
+  if (!(fgx >= 0.0f) || !(fgy >= 0.0f) || !(fgz >= 0.0f)) return false;
+  if (fgx >= float(P.sd[0] - 1u) || fgy >= float(P.sd[1] - 1u) || fgz >= float(P.sd[2] - 1u)) return false;
+  int i0 = int(fgx), j0 = int(fgy), k0 = int(fgz);
+  int i1 = i0 + 1, j1 = j0 + 1, k1 = k0 + 1;
+  float fx = gx - float(i0), fy = gy - float(j0), fz = gz - float(k0);
+  float c000 = sdf_node(P, i0, j0, k0), c001 = sdf_node(P, i0, j0, k1);
+  float c010 = sdf_node(P, i0, j1, k0), c011 = sdf_node(P, i0, j1, k1);
+  float c100 = sdf_node(P, i1, j0, k0), c101 = sdf_node(P, i1, j0, k1);
+  float c110 = sdf_node(P, i1, j1, k0), c111 = sdf_node(P, i1, j1, k1);
+  float c00 = c000 * (1.0f - fz) + c001 * fz;
+  float c01 = c010 * (1.0f - fz) + c011 * fz;
+  float c10 = c100 * (1.0f - fz) + c101 * fz;
+  float c11 = c110 * (1.0f - fz) + c111 * fz;
+  float c0 = c00 * (1.0f - fy) + c01 * fy;
+  float c1 = c10 * (1.0f - fy) + c11 * fy;
+  out = c0 * (1.0f - fx) + c1 * fx;
+  return true;
+}
+__device__ __forceinline__ bool sdf_gradient(const DevParams& P, float px, float py, float pz,
+                                             float& nx, float& ny, float& nz) {
+  float hh = P.scell * 0.5f;
+  float two_h = 2.0f * hh;
+  float a, b;
+  if (!sdf_sample(P, px + hh, py, pz, a) || !sdf_sample(P, px - hh, py, pz, b)) return false;
+  float dx = (a - b) / two_h;
+  if (!sdf_sample(P, px, py + hh, pz, a) || !sdf_sample(P, px, py - hh, pz, b)) return false;
+  float dy = (a - b) / two_h;
+  if (!sdf_sample(P, px, py, pz + hh, a) || !sdf_sample(P, px, py, pz - hh, b)) return false;
+  float dz = (a - b) / two_h;
+  float mag = sqrtf(dx * dx + dy * dy + dz * dz);
+  if (mag > 1e-8f) { nx = dx / mag; ny = dy / mag; nz = dz / mag; }
+  else { nx = 0.0f; ny = 0.0f; nz = 0.0f; }
+  return true;
+}
+// Wall record at a position: (n, d) with d = +inf when the wall cannot act
+// (outside the SDF grid, or d >= 2h so that no wall term is non-zero).
+__device__ __forceinline__ float4 wall_at(const DevParams& P, float px, float py, float pz) {
+  float4 w = make_float4(0.0f, 0.0f, 0.0f, __builtin_inff());
+  if (!P.sdf_on) return w;
+  float d;
+  if (!sdf_sample(P, px, py, pz, d)) return w;
+  float s = d / P.h;
+  if (!(s < 2.0f)) return w;
+  float nx, ny, nz;
+  if (!sdf_gradient(P, px, py, pz, nx, ny, nz)) return w;
+  return make_float4(nx, ny, nz, d);
+}
+
+// Wall integrals of the Wendland C2 kernel beyond a plane at distance s*h:
+// Omega(s) = (1-s/2)^6 (1/2 + 3/4 s + 3/8 s^2) and Gamma(s) = -dOmega/ds
+// = (1-s/2)^5 (3/4 + 15/8 s + 3/2 s^2).
+__device__ __forceinline__ float wall_omega(float s) {
+  if (s >= 2.0f) return 0.0f;
+  if (s < 0.0f) s = 0.0f;
+  float t = 1.0f - 0.5f * s;
+  float t2 = t * t;
+  float t4 = t2 * t2;
+  float t6 = t4 * t2;
+  float p = (0.375f * s + 0.75f) * s + 0.5f;
+  return t6 * p;
+}
+__device__ __forceinline__ float wall_gamma(float s) {
+  if (s >= 2.0f) return 0.0f;
+  if (s < 0.0f) s = 0.0f;
+  float t = 1.0f - 0.5f * s;
+  float t2 = t * t;
+  float t4 = t2 * t2;
+  float t5 = t4 * t;
+  float p = (1.5f * s + 1.875f) * s + 0.75f;
+  return t5 * p;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2); remap so XCD k runs a contiguous chunk of the spatially sorted work.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t nb) {
+  const uint32_t nx = 8u;
+  if (nb < nx) return b;
+  uint32_t per = (nb + nx - 1u) / nx;
+  uint32_t tall = nb % nx;
+  if (tall == 0u) tall = nx;
+  uint32_t xcd = b % nx, local = b / nx;
+  if (xcd < tall) return xcd * per + local;
+  return tall * per + (xcd - tall) * (per - 1u) + local;
+}
+
+}  // namespace ubp

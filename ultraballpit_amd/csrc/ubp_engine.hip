@@ -1,0 +1,840 @@
+// ubp_engine.hip -- host side of the MI355X WCSPH engine and its C ABI
+// (include/ubp.h).  One engine = one scene on one device, one HIP stream.
+// The engine mirrors the reference backend contract
+// (backend/crates/kernel/src/lib.rs:58-78 trait, gpu/mod.rs:190-1763 GpuKernel
+// extras); paths are relative to the reference repository.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ubp.h"
+#include "ubp_kernels.hip"
+
+namespace ubp {
+
+static thread_local std::string g_last_error;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess)                                                              \
+      throw Error(_e == hipErrorOutOfMemory ? UBP_ENOMEM : UBP_EHIP,                   \
+                  std::string(#expr) + ": " + hipGetErrorString(_e));                  \
+  } while (0)
+
+#define LAUNCHCHK(name)                                                                \
+  do {                                                                                 \
+    hipError_t _e = hipGetLastError();                                                 \
+    if (_e != hipSuccess) throw Error(UBP_EHIP, std::string(name) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+static uint32_t bits_for(uint32_t n) {
+  uint32_t b = 0;
+  while ((1u << b) < n) ++b;
+  return b;
+}
+// Morton interleave with per-axis widths: level l contributes x, y, z bits in
+// that order while each axis still has bits.
+static uint32_t morton_host(uint32_t cx, uint32_t cy, uint32_t cz, const uint32_t bits[3]) {
+  uint32_t code = 0, pos = 0;
+  uint32_t mb = std::max(bits[0], std::max(bits[1], bits[2]));
+  for (uint32_t l = 0; l < mb; ++l) {
+    if (l < bits[0]) { code |= ((cx >> l) & 1u) << pos; ++pos; }
+    if (l < bits[1]) { code |= ((cy >> l) & 1u) << pos; ++pos; }
+    if (l < bits[2]) { code |= ((cz >> l) & 1u) << pos; ++pos; }
+  }
+  return code;
+}
+
+template <class T> static void dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  HIPCHK(hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T)));
+}
+template <class T> static void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+struct Engine {
+  int dev = 0;
+  hipStream_t st = nullptr;
+  DevParams P{};
+  size_t n = 0;
+  uint32_t block = 256;
+  uint32_t bits[3] = {0, 0, 0};
+  uint32_t nblk_scan = 0;
+
+  float4 *pos = nullptr, *pos_t = nullptr, *vel = nullptr, *vel_t = nullptr, *acc = nullptr;
+  float4 *wall = nullptr, *wall_t = nullptr;
+  float *rho_g = nullptr, *rho_n = nullptr, *prs = nullptr, *prho2 = nullptr;
+  uint8_t* ft[2] = {nullptr, nullptr};
+  uint32_t* orig[2] = {nullptr, nullptr};
+  int pp = 0;
+  uint32_t *key = nullptr, *arr = nullptr, *cnt = nullptr, *start = nullptr, *bsum = nullptr;
+  uint32_t *stage_src = nullptr, *stage_orig = nullptr, *nbr = nullptr, *nn = nullptr;
+  uint32_t* spread = nullptr;
+  float* sdf = nullptr;
+  float* dtp = nullptr;
+  uint32_t* red = nullptr;
+  float* pmax = nullptr;
+  double* psum = nullptr;
+  uint32_t* dbg = nullptr;
+
+  // checkpoint
+  bool has_ckpt = false;
+  bool ck_needs_init = true;
+  int ck_pp = 0;
+  float4 *ck_pos = nullptr, *ck_vel = nullptr, *ck_acc = nullptr, *ck_wall = nullptr;
+  float *ck_rho = nullptr, *ck_prs = nullptr, *ck_prho2 = nullptr, *ck_rho_g = nullptr;
+  uint8_t* ck_ft = nullptr;
+  uint32_t *ck_orig = nullptr, *ck_start = nullptr;
+
+  // host mirror (original order)
+  std::vector<float> h_temp;
+  std::vector<uint8_t> h_ft;
+  std::vector<float> h_init[12];  // x y z vx vy vz ax ay az rho p m
+  bool needs_init = true;
+  bool dt_valid = false;
+  bool stepped = false;
+  double e0 = 0.0, m0 = 0.0;
+
+  // pinned staging
+  void* h_stage = nullptr;
+  size_t h_stage_bytes = 0;
+
+  // graph of two steady-state steps
+  hipGraphExec_t gexec = nullptr;
+  int graph_pp = -1;
+
+  ~Engine() { release(); }
+
+  void release() {
+    if (dev >= 0) (void)hipSetDevice(dev);
+    if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
+    if (st) { (void)hipStreamSynchronize(st); }
+    dfree(pos); dfree(pos_t); dfree(vel); dfree(vel_t); dfree(acc); dfree(wall); dfree(wall_t);
+    dfree(rho_g); dfree(rho_n); dfree(prs); dfree(prho2);
+    for (int k = 0; k < 2; ++k) { dfree(ft[k]); dfree(orig[k]); }
+    dfree(key); dfree(arr); dfree(cnt); dfree(start); dfree(bsum); dfree(stage_src);
+    dfree(stage_orig); dfree(nbr); dfree(nn); dfree(spread); dfree(sdf); dfree(dtp); dfree(red);
+    dfree(pmax); dfree(psum); dfree(dbg);
+    dfree(ck_pos); dfree(ck_vel); dfree(ck_acc); dfree(ck_wall); dfree(ck_rho); dfree(ck_prs);
+    dfree(ck_prho2); dfree(ck_rho_g); dfree(ck_ft); dfree(ck_orig); dfree(ck_start);
+    if (h_stage) { (void)hipHostFree(h_stage); h_stage = nullptr; }
+    if (st) { (void)hipStreamDestroy(st); st = nullptr; }
+  }
+
+  uint32_t grid_of(size_t count) const { return uint32_t((count + block - 1) / block); }
+
+  void set_derived(float alpha, float beta) {
+    const float h = P.h;
+    const float pi_f = 3.14159265358979323846f;
+    const float sixteen_pi = 16.0f * pi_f;
+    const float wn = 21.0f / sixteen_pi;             // sph.rs:20
+    const float h3 = h * h * h;
+    P.norm_h3 = wn / h3;
+    P.norm_h4 = wn / (h3 * h);
+    const float sr = 2.0f * h;
+    P.rsq = sr * sr;                                 // neighbor.rs:144
+    P.eta_sq = 0.01f * h * h;
+    P.diff_coef = 0.1f * h * P.cs;
+    P.nac = -alpha * P.cs;
+    P.beta = beta;
+    P.r0 = 0.5f * h;
+    P.d_rep = 10.0f * 9.81f * 0.01f;
+    const float rho0[2] = {1000.0f, 1.204f};
+    const float gam[2] = {7.0f, 1.4f};
+    for (int f = 0; f < 2; ++f) {
+      P.rho0[f] = rho0[f];
+      P.gamma[f] = gam[f];
+      P.b_tait[f] = rho0[f] * P.cs * P.cs / gam[f];  // eos.rs:40
+    }
+  }
+
+  void create(const ubp_particles_in* pin, const ubp_boundary_in* bin, float h, const float g[3],
+              float cs, float cfl, const float dmin[3], const float dmax[3]) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) throw Error(UBP_ENODEV, "no HIP device visible");
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+      throw Error(UBP_ENODEV, std::string("device is ") + prop.gcnArchName + ", this build targets gfx950");
+    if (bin && bin->n > 0)
+      throw Error(UBP_EINVAL, "boundary particles are not supported by this build; use ubp_set_sdf walls");
+    if (!(h > 0.0f)) throw Error(UBP_EINVAL, "h must be positive");
+    n = pin ? pin->n : 0;
+    if (n > (size_t(1) << 28)) throw Error(UBP_EOVERFLOW, "too many particles (max 2^28)");
+    HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+
+    P.n = uint32_t(n);
+    P.K = 128;
+    P.h = h;
+    P.cs = cs;
+    P.cfl = cfl;
+    for (int a = 0; a < 3; ++a) { P.g[a] = g[a]; P.dmin[a] = dmin[a]; P.dmax[a] = dmax[a]; }
+    P.cell = 2.0f * h;  // lib.rs:149
+    uint64_t codes = 1;
+    for (int a = 0; a < 3; ++a) {
+      float d = std::ceil((dmax[a] - dmin[a]) / P.cell);  // neighbor.rs:31-35
+      if (!(d >= 1.0f)) d = 1.0f;
+      if (d > 65536.0f) throw Error(UBP_EOVERFLOW, "grid too large");
+      P.dims[a] = uint32_t(d);
+      bits[a] = bits_for(P.dims[a]);
+      codes <<= bits[a];
+    }
+    if (codes > (uint64_t(1) << 28)) throw Error(UBP_EOVERFLOW, "Morton code space exceeds 2^28 cells");
+    P.ncodes = uint32_t(codes);
+    P.diff_floor = 0.0f;
+    P.sdf_on = 0;
+    set_derived(1.0f, 2.0f);  // reference alpha/beta (sph.rs:440-441)
+
+    // Morton spread tables
+    std::vector<uint32_t> sp(P.dims[0] + P.dims[1] + P.dims[2]);
+    for (uint32_t c = 0; c < P.dims[0]; ++c) sp[c] = morton_host(c, 0, 0, bits);
+    for (uint32_t c = 0; c < P.dims[1]; ++c) sp[P.dims[0] + c] = morton_host(0, c, 0, bits);
+    for (uint32_t c = 0; c < P.dims[2]; ++c) sp[P.dims[0] + P.dims[1] + c] = morton_host(0, 0, c, bits);
+    dalloc(&spread, sp.size());
+    HIPCHK(hipMemcpy(spread, sp.data(), sp.size() * 4, hipMemcpyHostToDevice));
+    P.spread = spread;
+
+    // host mirror
+    const float* src[12] = {pin ? pin->x : nullptr, pin ? pin->y : nullptr, pin ? pin->z : nullptr,
+                            pin ? pin->vx : nullptr, pin ? pin->vy : nullptr, pin ? pin->vz : nullptr,
+                            pin ? pin->ax : nullptr, pin ? pin->ay : nullptr, pin ? pin->az : nullptr,
+                            pin ? pin->density : nullptr, nullptr, pin ? pin->mass : nullptr};
+    if (n && (!pin->x || !pin->y || !pin->z || !pin->mass || !pin->density))
+      throw Error(UBP_EINVAL, "x, y, z, mass and density are required");
+    for (int k = 0; k < 12; ++k) {
+      h_init[k].assign(n, 0.0f);
+      if (src[k]) std::memcpy(h_init[k].data(), src[k], n * 4);
+    }
+    h_temp.assign(n, 293.15f);
+    if (pin && pin->temperature) std::memcpy(h_temp.data(), pin->temperature, n * 4);
+    h_ft.assign(n, 0);
+    if (pin && pin->fluid_type) std::memcpy(h_ft.data(), pin->fluid_type, n);
+    for (size_t i = 0; i < n; ++i)
+      if (h_ft[i] > 1) throw Error(UBP_EINVAL, "fluid_type must be 0 (Water) or 1 (Air)");
+
+    // initial energy / mass (lib.rs:152-156), sequential FP64 in original order
+    e0 = 0.0; m0 = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+      double m = h_init[11][i];
+      double vx = h_init[3][i], vy = h_init[4][i], vz = h_init[5][i];
+      e0 += 0.5 * m * (vx * vx + vy * vy + vz * vz);
+      e0 -= m * (double(g[0]) * h_init[0][i] + double(g[1]) * h_init[1][i] + double(g[2]) * h_init[2][i]);
+      m0 += m;
+    }
+
+    // device state
+    dalloc(&pos, n); dalloc(&pos_t, n); dalloc(&vel, n); dalloc(&vel_t, n); dalloc(&acc, n);
+    dalloc(&wall, n); dalloc(&wall_t, n);
+    dalloc(&rho_g, n); dalloc(&rho_n, n); dalloc(&prs, n); dalloc(&prho2, n);
+    for (int k = 0; k < 2; ++k) { dalloc(&ft[k], n); dalloc(&orig[k], n); }
+    dalloc(&key, n); dalloc(&arr, n); dalloc(&stage_src, n); dalloc(&stage_orig, n); dalloc(&nn, n);
+    dalloc(&cnt, size_t(P.ncodes) + 1); dalloc(&start, size_t(P.ncodes) + 1);
+    nblk_scan = (P.ncodes + 1023u) / 1024u;
+    dalloc(&bsum, nblk_scan);
+    alloc_nbr();
+    dalloc(&dtp, 4); dalloc(&red, 4);
+    dalloc(&pmax, (n + 255) / 256 + 1); dalloc(&psum, 2 * ((n + 255) / 256 + 1));
+    HIPCHK(hipMemset(cnt, 0, (size_t(P.ncodes) + 1) * 4));
+    HIPCHK(hipMemset(red, 0, 16));
+    HIPCHK(hipMemset(dtp, 0, 16));
+    uint32_t total = uint32_t(n);
+    HIPCHK(hipMemcpy(start + P.ncodes, &total, 4, hipMemcpyHostToDevice));  // start[ncodes] = N
+
+    // upload the initial state (original order) into the staging buffers
+    std::vector<float4> tmp(n);
+    for (size_t i = 0; i < n; ++i) tmp[i] = make_float4(h_init[0][i], h_init[1][i], h_init[2][i], h_init[11][i]);
+    if (n) HIPCHK(hipMemcpy(pos_t, tmp.data(), n * 16, hipMemcpyHostToDevice));
+    for (size_t i = 0; i < n; ++i) tmp[i] = make_float4(h_init[3][i], h_init[4][i], h_init[5][i], 0.0f);
+    if (n) HIPCHK(hipMemcpy(vel_t, tmp.data(), n * 16, hipMemcpyHostToDevice));
+    for (size_t i = 0; i < n; ++i) tmp[i] = make_float4(h_init[6][i], h_init[7][i], h_init[8][i], 0.0f);
+    if (n) HIPCHK(hipMemcpy(acc, tmp.data(), n * 16, hipMemcpyHostToDevice));
+    if (n) HIPCHK(hipMemcpy(rho_n, h_init[9].data(), n * 4, hipMemcpyHostToDevice));
+    if (n) HIPCHK(hipMemcpy(ft[pp], h_ft.data(), n, hipMemcpyHostToDevice));
+    std::vector<uint32_t> iota(n);
+    for (size_t i = 0; i < n; ++i) iota[i] = uint32_t(i);
+    if (n) HIPCHK(hipMemcpy(orig[pp], iota.data(), n * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipDeviceSynchronize());
+  }
+
+  void alloc_nbr() {
+    dfree(nbr);
+    size_t cap = size_t(P.K) * n;
+    dalloc(&nbr, cap);
+  }
+
+  void invalidate_graph() {
+    if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
+    graph_pp = -1;
+  }
+
+  void check_not_started(const char* what) {
+    if (stepped) throw Error(UBP_EINVAL, std::string(what) + " must be called before the first step");
+  }
+
+  void set_sdf(const float o[3], float cellsz, const uint32_t d[3], const float* data) {
+    check_not_started("ubp_set_sdf");
+    if (!(cellsz > 0.0f) || d[0] < 2 || d[1] < 2 || d[2] < 2 || !data)
+      throw Error(UBP_EINVAL, "invalid SDF grid");
+    size_t nn_ = size_t(d[0]) * d[1] * d[2];
+    if (nn_ > (size_t(1) << 31)) throw Error(UBP_EOVERFLOW, "SDF grid too large");
+    dfree(sdf);
+    dalloc(&sdf, nn_);
+    HIPCHK(hipMemcpy(sdf, data, nn_ * 4, hipMemcpyHostToDevice));
+    for (int a = 0; a < 3; ++a) { P.so[a] = o[a]; P.sd[a] = d[a]; }
+    P.scell = cellsz;
+    P.sdf = sdf;
+    P.sdf_on = 1;
+    invalidate_graph();
+  }
+
+  void set_visc(float alpha, float beta) {
+    check_not_started("ubp_set_viscosity_coeffs");
+    set_derived(alpha, beta);
+    invalidate_graph();
+  }
+
+  void set_tuning(uint32_t bs, uint32_t K) {
+    if (bs) {
+      if (bs != 64 && bs != 128 && bs != 256) throw Error(UBP_EINVAL, "block size must be 64, 128 or 256");
+      block = bs;
+    }
+    if (K && K != P.K) {
+      if (K > 1024) throw Error(UBP_EINVAL, "neighbour capacity must be <= 1024");
+      HIPCHK(hipStreamSynchronize(st));
+      P.K = K;
+      alloc_nbr();
+      if (!needs_init) launch(k_neigh, grid_of(n), P, pos, start, nbr, nn);  // rebuild lists
+    }
+    invalidate_graph();
+  }
+
+  template <class Kern, class... A> void launch(Kern k, uint32_t grid, A... args) {
+    if (grid == 0) return;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(block), 0, st, args...);
+    LAUNCHCHK("kernel launch");
+  }
+
+  // counting sort of pos_t order into slot order (keys/arrivals already made)
+  void bin() {
+    hipLaunchKernelGGL(k_scan_local, dim3(nblk_scan), dim3(256), 0, st, cnt, start, bsum, P.ncodes);
+    LAUNCHCHK("k_scan_local");
+    hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(256), 0, st, bsum, nblk_scan);
+    LAUNCHCHK("k_scan_bsum");
+    hipLaunchKernelGGL(k_scan_add, dim3(nblk_scan), dim3(256), 0, st, start, bsum, P.ncodes);
+    LAUNCHCHK("k_scan_add");
+    const uint32_t g = grid_of(n);
+    launch(k_scatter, g, P, key, arr, start, orig[pp], stage_src, stage_orig);
+    launch(k_rank_gather, g, P, stage_src, stage_orig, key, start, pos_t, vel_t, rho_n, wall_t, ft[pp],
+           pos, vel, rho_g, wall, ft[pp ^ 1], orig[pp ^ 1]);
+    pp ^= 1;
+  }
+
+  void density_forces() {
+    const uint32_t g = grid_of(n);
+    launch(k_neigh, g, P, pos, start, nbr, nn);
+    launch(k_density, g, P, pos, rho_g, wall, ft[pp], start, nbr, nn, dtp, rho_n, prs, prho2);
+    launch(k_forces, g, P, pos, vel, rho_n, prs, prho2, wall, ft[pp], start, nbr, nn, acc);
+  }
+
+  // CpuKernel::compute_forces on the initial state (lib.rs:295-298)
+  void bootstrap() {
+    launch(k_key_count, grid_of(n), P, pos_t, key, arr, cnt, wall_t);
+    bin();
+    density_forces();
+    needs_init = false;
+  }
+
+  void step_body(int do_cfl) {
+    const uint32_t g = grid_of(n);
+    launch(k_kick, g, P, vel, acc, dtp);
+    launch(k_xsph_drift, g, P, pos, vel, rho_n, start, nbr, nn, dtp, pos_t, vel_t, wall_t, key, arr, cnt);
+    bin();
+    density_forces();
+    launch(k_kick2_cfl, g, P, vel, acc, dtp, red, 1, do_cfl);
+  }
+
+  void set_dt(float dt) {
+    hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(64), 0, st, dtp, dt);
+    LAUNCHCHK("k_set_dt");
+  }
+
+  void step_host_dt(float dt) {
+    if (n == 0) return;
+    stepped = true;
+    set_dt(dt);
+    if (needs_init) bootstrap();
+    step_body(0);
+    dt_valid = false;
+  }
+
+  void ensure_dt() {
+    if (dt_valid) return;
+    // compute_timestep on the current state (initial arrays before bootstrap)
+    const float4* v = needs_init ? vel_t : vel;
+    hipLaunchKernelGGL(k_kick2_cfl, dim3(grid_of(n)), dim3(block), 0, st, P, (float4*)v, (const float4*)acc,
+                       dtp, red, 0, 1);
+    LAUNCHCHK("k_kick2_cfl(cfl only)");
+    dt_valid = true;
+  }
+
+  void step_device_dt(int nsteps, bool use_graph) {
+    if (n == 0 || nsteps <= 0) return;
+    stepped = true;
+    ensure_dt();
+    int k = 0;
+    if (needs_init) { bootstrap(); step_body(1); ++k; }
+    if (use_graph && nsteps - k >= 2) {
+      if (!gexec || graph_pp != pp) build_graph();
+      while (nsteps - k >= 2) {
+        HIPCHK(hipGraphLaunch(gexec, st));
+        k += 2;
+      }
+    }
+    for (; k < nsteps; ++k) step_body(1);
+  }
+
+  void build_graph() {
+    invalidate_graph();
+    hipGraph_t g;
+    int pp0 = pp;
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+      step_body(1);
+      step_body(1);
+    } catch (...) {
+      hipGraph_t dummy;
+      (void)hipStreamEndCapture(st, &dummy);
+      pp = pp0;
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(st, &g));
+    hipError_t e = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) { gexec = nullptr; throw Error(UBP_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)); }
+    // capture recorded two flips of the ping-pong index; replay leaves it there
+    if (pp != pp0) throw Error(UBP_EHIP, "graph capture: ping-pong mismatch");
+    graph_pp = pp0;
+  }
+
+  void sync() { HIPCHK(hipStreamSynchronize(st)); }
+
+  void ensure_stage(size_t bytes) {
+    if (h_stage_bytes >= bytes) return;
+    if (h_stage) (void)hipHostFree(h_stage);
+    h_stage = nullptr;
+    h_stage_bytes = 0;
+    HIPCHK(hipHostMalloc(&h_stage, bytes));
+    h_stage_bytes = bytes;
+  }
+
+  void read(const ubp_particles_out* o) {
+    sync();
+    if (needs_init) {
+      float* dst[13] = {o->x, o->y, o->z, o->vx, o->vy, o->vz, o->ax, o->ay, o->az, o->density,
+                        o->pressure, o->mass, o->temperature};
+      for (int k = 0; k < 12; ++k)
+        if (dst[k]) std::memcpy(dst[k], h_init[k].data(), n * 4);
+      if (o->temperature) std::memcpy(o->temperature, h_temp.data(), n * 4);
+      if (o->fluid_type) std::memcpy(o->fluid_type, h_ft.data(), n);
+      return;
+    }
+    // pos, vel, acc (16 B each), rho, p (4 B), orig (4 B)
+    size_t bytes = n * (16 * 3 + 4 * 3);
+    ensure_stage(bytes);
+    char* base = static_cast<char*>(h_stage);
+    float4* hp = reinterpret_cast<float4*>(base);
+    float4* hv = hp + n;
+    float4* ha = hv + n;
+    float* hr = reinterpret_cast<float*>(ha + n);
+    float* hpr = hr + n;
+    uint32_t* ho = reinterpret_cast<uint32_t*>(hpr + n);
+    HIPCHK(hipMemcpyAsync(hp, pos, n * 16, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hv, vel, n * 16, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(ha, acc, n * 16, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hr, rho_n, n * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hpr, prs, n * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(ho, orig[pp], n * 4, hipMemcpyDeviceToHost, st));
+    sync();
+    for (size_t s = 0; s < n; ++s) {
+      uint32_t i = ho[s];
+      if (o->x) o->x[i] = hp[s].x;
+      if (o->y) o->y[i] = hp[s].y;
+      if (o->z) o->z[i] = hp[s].z;
+      if (o->mass) o->mass[i] = hp[s].w;
+      if (o->vx) o->vx[i] = hv[s].x;
+      if (o->vy) o->vy[i] = hv[s].y;
+      if (o->vz) o->vz[i] = hv[s].z;
+      if (o->ax) o->ax[i] = ha[s].x;
+      if (o->ay) o->ay[i] = ha[s].y;
+      if (o->az) o->az[i] = ha[s].z;
+      if (o->density) o->density[i] = hr[s];
+      if (o->pressure) o->pressure[i] = hpr[s];
+    }
+    if (o->temperature) std::memcpy(o->temperature, h_temp.data(), n * 4);
+    if (o->fluid_type) std::memcpy(o->fluid_type, h_ft.data(), n);
+  }
+
+  void metrics(ubp_error_metrics_t* out) {
+    if (n == 0) { out->max_density_variation = 0; out->energy_conservation = 0; out->mass_conservation = 0; return; }
+    const uint32_t g = uint32_t((n + 255) / 256);  // k_metrics is written for 256 threads
+    const float4* p = needs_init ? pos_t : pos;
+    const float4* v = needs_init ? vel_t : vel;
+    hipLaunchKernelGGL(k_metrics, dim3(g), dim3(256), 0, st, P, p, v, (const float*)rho_n,
+                       (const uint8_t*)ft[pp], pmax, psum);
+    LAUNCHCHK("k_metrics");
+    std::vector<float> hm(g);
+    std::vector<double> hs(2 * size_t(g));
+    HIPCHK(hipMemcpyAsync(hm.data(), pmax, g * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hs.data(), psum, 2 * size_t(g) * 8, hipMemcpyDeviceToHost, st));
+    sync();
+    float mx = 0.0f;
+    double e = 0.0, m = 0.0;
+    for (uint32_t b = 0; b < g; ++b) {
+      if (hm[b] > mx) mx = hm[b];
+      e += hs[2 * b];
+      m += hs[2 * b + 1];
+    }
+    out->max_density_variation = mx;
+    out->energy_conservation = (std::fabs(e0) > 1.0e-12) ? float(std::fabs((e - e0) / e0)) : float(std::fabs(e - e0));
+    out->mass_conservation = (std::fabs(m0) > 1.0e-12) ? float(std::fabs((m - m0) / m0)) : float(std::fabs(m - m0));
+  }
+
+  void debug_bins(uint32_t* keys, uint32_t* perm, uint32_t* counts) {
+    if (needs_init) throw Error(UBP_EINVAL, "no binning before the first step");
+    if (!dbg) dalloc(&dbg, n);
+    const uint32_t g = grid_of(n);
+    std::vector<uint32_t> ho(n), hv(n);
+    HIPCHK(hipMemcpyAsync(ho.data(), orig[pp], n * 4, hipMemcpyDeviceToHost, st));
+    if (keys) {
+      launch(k_debug_keys, g, P, pos, orig[pp], dbg);
+      HIPCHK(hipMemcpyAsync(keys, dbg, n * 4, hipMemcpyDeviceToHost, st));
+    }
+    if (counts) HIPCHK(hipMemcpyAsync(hv.data(), nn, n * 4, hipMemcpyDeviceToHost, st));
+    sync();
+    if (perm) std::memcpy(perm, ho.data(), n * 4);
+    if (counts)
+      for (size_t s = 0; s < n; ++s) counts[ho[s]] = hv[s];
+  }
+
+  void save_ckpt() {
+    if (!ck_pos) {
+      dalloc(&ck_pos, n); dalloc(&ck_vel, n); dalloc(&ck_acc, n); dalloc(&ck_wall, n);
+      dalloc(&ck_rho, n); dalloc(&ck_prs, n); dalloc(&ck_prho2, n); dalloc(&ck_rho_g, n);
+      dalloc(&ck_ft, n); dalloc(&ck_orig, n); dalloc(&ck_start, size_t(P.ncodes) + 1);
+    }
+    auto cp = [&](void* d, const void* s, size_t b) { if (b) HIPCHK(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, st)); };
+    const bool pre = needs_init;
+    cp(ck_pos, pre ? pos_t : pos, n * 16);
+    cp(ck_vel, pre ? vel_t : vel, n * 16);
+    cp(ck_acc, acc, n * 16);
+    cp(ck_wall, wall, n * 16);
+    cp(ck_rho, rho_n, n * 4);
+    cp(ck_prs, prs, n * 4);
+    cp(ck_prho2, prho2, n * 4);
+    cp(ck_rho_g, rho_g, n * 4);
+    cp(ck_ft, ft[pp], n);
+    cp(ck_orig, orig[pp], n * 4);
+    cp(ck_start, start, (size_t(P.ncodes) + 1) * 4);
+    ck_needs_init = needs_init;
+    ck_pp = pp;
+    has_ckpt = true;
+  }
+
+  bool restore_ckpt() {
+    if (!has_ckpt) return false;
+    auto cp = [&](void* d, const void* s, size_t b) { if (b) HIPCHK(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, st)); };
+    pp = ck_pp;
+    needs_init = ck_needs_init;
+    cp(needs_init ? pos_t : pos, ck_pos, n * 16);
+    cp(needs_init ? vel_t : vel, ck_vel, n * 16);
+    cp(acc, ck_acc, n * 16);
+    cp(wall, ck_wall, n * 16);
+    cp(rho_n, ck_rho, n * 4);
+    cp(prs, ck_prs, n * 4);
+    cp(prho2, ck_prho2, n * 4);
+    cp(rho_g, ck_rho_g, n * 4);
+    cp(ft[pp], ck_ft, n);
+    cp(orig[pp], ck_orig, n * 4);
+    cp(start, ck_start, (size_t(P.ncodes) + 1) * 4);
+    if (!needs_init) launch(k_neigh, grid_of(n), P, pos, start, nbr, nn);  // lists of the restored grid
+    has_ckpt = false;
+    dt_valid = false;
+    invalidate_graph();
+    sync();
+    return true;
+  }
+};
+
+static int fail(const Error& e) {
+  g_last_error = e.what();
+  return e.code;
+}
+
+}  // namespace ubp
+
+using ubp::Engine;
+using ubp::Error;
+
+struct ubp_engine {
+  Engine eng;
+};
+
+#define UBP_API_BEGIN                 \
+  ubp::g_last_error.clear();          \
+  try {
+#define UBP_API_END                                        \
+  }                                                        \
+  catch (const Error& e) { return ubp::fail(e); }          \
+  catch (const std::exception& e) {                        \
+    ubp::g_last_error = e.what();                          \
+    return UBP_EHIP;                                       \
+  }                                                        \
+  catch (...) {                                            \
+    ubp::g_last_error = "unknown error";                   \
+    return UBP_EHIP;                                       \
+  }
+
+#define UBP_HANDLE_CHECK(e)                                                        \
+  if (!(e)) throw Error(UBP_EINVAL, "null engine handle");                         \
+  { hipError_t _s = hipSetDevice((e)->eng.dev);                                    \
+    if (_s != hipSuccess) throw Error(UBP_EHIP, std::string("hipSetDevice: ") + hipGetErrorString(_s)); }
+
+extern "C" {
+
+ubp_engine* ubp_create(const ubp_particles_in* particles, const ubp_boundary_in* boundary, float h,
+                       const float gravity[3], float speed_of_sound, float cfl_number, float viscosity,
+                       const float domain_min[3], const float domain_max[3]) {
+  (void)viscosity;  // ignored like the reference (lib.rs:107-109)
+  ubp::g_last_error.clear();
+  ubp_engine* e = nullptr;
+  try {
+    if (!gravity || !domain_min || !domain_max) throw Error(UBP_EINVAL, "null gravity/domain");
+    e = new ubp_engine();
+    e->eng.create(particles, boundary, h, gravity, speed_of_sound, cfl_number, domain_min, domain_max);
+    return e;
+  } catch (const Error& err) {
+    ubp::g_last_error = err.what();
+  } catch (const std::exception& err) {
+    ubp::g_last_error = err.what();
+  }
+  delete e;
+  return nullptr;
+}
+
+void ubp_destroy(ubp_engine* e) {
+  if (!e) return;
+  try { delete e; } catch (...) {}
+}
+
+const char* ubp_last_error(void) { return ubp::g_last_error.c_str(); }
+
+int ubp_gpu_available(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return 0;
+  for (int d = 0; d < n; ++d) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, d) == hipSuccess && std::strncmp(p.gcnArchName, "gfx950", 6) == 0) return 1;
+  }
+  return 0;
+}
+
+int ubp_set_viscosity_coeffs(ubp_engine* e, float alpha, float beta) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  e->eng.set_visc(alpha, beta);
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_set_sdf(ubp_engine* e, const float origin[3], float cell, const uint32_t dims[3], const float* d) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  if (!origin || !dims) throw Error(UBP_EINVAL, "null SDF argument");
+  e->eng.set_sdf(origin, cell, dims, d);
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_set_tuning(ubp_engine* e, uint32_t block_size, uint32_t nbr_capacity) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  e->eng.set_tuning(block_size, nbr_capacity);
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_step(ubp_engine* e, float dt) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  e->eng.step_host_dt(dt);
+  e->eng.sync();
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_step_async(ubp_engine* e, float dt) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  e->eng.step_host_dt(dt);
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_sync(ubp_engine* e) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  e->eng.sync();
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_step_n(ubp_engine* e, int nsteps, int use_graph) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  if (nsteps < 0) throw Error(UBP_EINVAL, "nsteps < 0");
+  e->eng.step_device_dt(nsteps, use_graph != 0);
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_get_dt(ubp_engine* e, float* dt) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  if (!dt) throw Error(UBP_EINVAL, "null dt");
+  e->eng.ensure_dt();
+  HIPCHK(hipMemcpyAsync(dt, e->eng.dtp, 4, hipMemcpyDeviceToHost, e->eng.st));
+  e->eng.sync();
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_step_profiled(ubp_engine* e, float dt, ubp_step_profile* out) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  if (!out) throw Error(UBP_EINVAL, "null profile");
+  Engine& g = e->eng;
+  std::memset(out, 0, sizeof(*out));
+  if (g.n == 0) return UBP_OK;
+  auto t0 = std::chrono::steady_clock::now();
+  g.stepped = true;
+  g.set_dt(dt);
+  if (g.needs_init) { g.bootstrap(); }
+  hipEvent_t ev[7];
+  for (auto& x : ev) HIPCHK(hipEventCreate(&x));
+  const uint32_t gr = g.grid_of(g.n);
+  HIPCHK(hipEventRecord(ev[0], g.st));
+  g.launch(ubp::k_kick, gr, g.P, g.vel, g.acc, g.dtp);
+  g.launch(ubp::k_xsph_drift, gr, g.P, g.pos, g.vel, g.rho_n, g.start, g.nbr, g.nn, g.dtp, g.pos_t,
+           g.vel_t, g.wall_t, g.key, g.arr, g.cnt);
+  HIPCHK(hipEventRecord(ev[1], g.st));
+  g.bin();
+  HIPCHK(hipEventRecord(ev[2], g.st));
+  g.launch(ubp::k_neigh, gr, g.P, g.pos, g.start, g.nbr, g.nn);
+  g.launch(ubp::k_density, gr, g.P, g.pos, g.rho_g, g.wall, g.ft[g.pp], g.start, g.nbr, g.nn, g.dtp,
+           g.rho_n, g.prs, g.prho2);
+  HIPCHK(hipEventRecord(ev[3], g.st));
+  g.launch(ubp::k_forces, gr, g.P, g.pos, g.vel, g.rho_n, g.prs, g.prho2, g.wall, g.ft[g.pp], g.start,
+           g.nbr, g.nn, g.acc);
+  HIPCHK(hipEventRecord(ev[4], g.st));
+  g.launch(ubp::k_kick2_cfl, gr, g.P, g.vel, g.acc, g.dtp, g.red, 1, 0);
+  HIPCHK(hipEventRecord(ev[5], g.st));
+  g.sync();
+  g.dt_valid = false;
+  float ms[5];
+  for (int k = 0; k < 5; ++k) HIPCHK(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
+  auto t1 = std::chrono::steady_clock::now();
+  // readback leg of the reference profile: a full particles() copy
+  {
+    std::vector<float> buf(g.n * 12);
+    ubp_particles_out o{};
+    float* b = buf.data();
+    o.x = b; o.y = b + g.n; o.z = b + 2 * g.n; o.vx = b + 3 * g.n; o.vy = b + 4 * g.n; o.vz = b + 5 * g.n;
+    o.ax = b + 6 * g.n; o.ay = b + 7 * g.n; o.az = b + 8 * g.n; o.density = b + 9 * g.n;
+    o.pressure = b + 10 * g.n; o.mass = b + 11 * g.n;
+    g.read(&o);
+  }
+  auto t2 = std::chrono::steady_clock::now();
+  for (auto& x : ev) (void)hipEventDestroy(x);
+  out->integrate_us = uint64_t((ms[0] + ms[4]) * 1000.0f);
+  out->grid_build_us = uint64_t(ms[1] * 1000.0f);
+  out->density_us = uint64_t(ms[2] * 1000.0f);
+  out->forces_us = uint64_t(ms[3] * 1000.0f);
+  out->boundary_pressure_us = 0;
+  out->readback_us = uint64_t(std::chrono::duration<double, std::micro>(t2 - t1).count());
+  out->total_us = uint64_t(std::chrono::duration<double, std::micro>(t2 - t0).count());
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_read_particles(ubp_engine* e, const ubp_particles_out* out) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  if (!out) throw Error(UBP_EINVAL, "null output view");
+  e->eng.read(out);
+  return UBP_OK;
+  UBP_API_END
+}
+
+size_t ubp_particle_count(const ubp_engine* e) { return e ? e->eng.n : 0; }
+
+int ubp_error_metrics(ubp_engine* e, ubp_error_metrics_t* out) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  if (!out) throw Error(UBP_EINVAL, "null metrics");
+  e->eng.metrics(out);
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_save_checkpoint(ubp_engine* e) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  e->eng.save_ckpt();
+  return 1;
+  UBP_API_END
+}
+
+int ubp_restore_checkpoint(ubp_engine* e) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  return e->eng.restore_ckpt() ? 1 : 0;
+  UBP_API_END
+}
+
+int ubp_debug_bins(ubp_engine* e, uint32_t* keys, uint32_t* perm, uint32_t* nbr_counts) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  e->eng.debug_bins(keys, perm, nbr_counts);
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_grid_info(const ubp_engine* e, uint32_t dims[3], uint32_t bits[3]) {
+  UBP_API_BEGIN
+  if (!e) throw Error(UBP_EINVAL, "null engine handle");
+  for (int a = 0; a < 3; ++a) {
+    if (dims) dims[a] = e->eng.P.dims[a];
+    if (bits) bits[a] = e->eng.bits[a];
+  }
+  return UBP_OK;
+  UBP_API_END
+}
+
+}  // extern "C"

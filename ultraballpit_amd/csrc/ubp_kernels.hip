@@ -1,0 +1,536 @@
+// ubp_kernels.hip -- CDNA4 (gfx950) kernels of the WCSPH step.
+//
+// Data layout in HBM (slot order = sorted by (Morton cell key, original id)):
+//   pos   float4 (x, y, z, m)      vel  float4 (vx, vy, vz, 0)
+//   acc   float4 (ax, ay, az, 0)   wall float4 (nx, ny, nz, d) SDF record
+//   rho_g old density (gathered)   rho_n new density, prs, prho2 = P/rho^2
+//   ft u8, orig u32 (ping-pong)    nbr u32 [K][N] column-major neighbour lists
+// One step (paths relative to the reference repository; CPU step
+// backend/crates/kernel/src/lib.rs:287-335):
+//   kick1 -> xsph_drift (+ SDF projection, domain clamp, cell key, histogram)
+//   -> scan -> scatter -> rank_gather -> neigh -> density -> forces
+//   -> kick2 (+ CFL reduction for the next dt, dt stays on the device).
+#include "ubp_device.hpp"
+
+namespace ubp {
+
+#define UBP_LB __launch_bounds__(256)
+
+__global__ void k_set_dt(float* dtp, float dt) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) dtp[0] = dt;
+}
+
+// Cell key + histogram for positions in the current (unsorted) order; also the
+// SDF wall record at those positions.  Used for the bootstrap binning.
+__global__ UBP_LB void k_key_count(DevParams P, const float4* __restrict__ pos_t,
+                                   uint32_t* __restrict__ key, uint32_t* __restrict__ arr,
+                                   uint32_t* __restrict__ cnt, float4* __restrict__ wall_t) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  float4 p = pos_t[i];
+  uint32_t cx, cy, cz;
+  pos_to_cell(P, p.x, p.y, p.z, cx, cy, cz);
+  uint32_t code = morton(P, cx, cy, cz);
+  key[i] = code;
+  arr[i] = atomicAdd(&cnt[code], 1u);
+  wall_t[i] = wall_at(P, p.x, p.y, p.z);
+}
+
+// ---- exclusive scan of the per-code histogram (3 phases) ----
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+// 256-thread block exclusive scan; returns exclusive prefix, sets total.
+__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* sh, uint32_t& total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t inc = wave_incl_scan(v);
+  if (lane == 63) sh[w] = inc;
+  __syncthreads();
+  uint32_t base = 0;
+  for (int k = 0; k < w; ++k) base += sh[k];
+  total = sh[0] + sh[1] + sh[2] + sh[3];
+  __syncthreads();
+  return base + inc - v;
+}
+
+// each block scans 1024 codes (4 per thread); zeroes the histogram for the
+// next step; writes its total to bsum
+__global__ UBP_LB void k_scan_local(uint32_t* __restrict__ cnt, uint32_t* __restrict__ start,
+                                    uint32_t* __restrict__ bsum, uint32_t ncodes) {
+  __shared__ uint32_t sh[4];
+  uint32_t base = blockIdx.x * 1024u + threadIdx.x * 4u;
+  uint32_t c[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) c[k] = (base + k < ncodes) ? cnt[base + k] : 0u;
+  uint32_t s = c[0] + c[1] + c[2] + c[3];
+  uint32_t total;
+  uint32_t ex = block_excl_scan256(s, sh, total);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (base + k < ncodes) { start[base + k] = ex; cnt[base + k] = 0u; }
+    ex += c[k];
+  }
+  if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+// single block: exclusive scan of bsum in place
+__global__ UBP_LB void k_scan_bsum(uint32_t* __restrict__ bsum, uint32_t nb) {
+  __shared__ uint32_t sh[4];
+  uint32_t carry = 0;
+  for (uint32_t off = 0; off < nb; off += 256u) {
+    uint32_t i = off + threadIdx.x;
+    uint32_t v = (i < nb) ? bsum[i] : 0u;
+    uint32_t total;
+    uint32_t ex = block_excl_scan256(v, sh, total);
+    if (i < nb) bsum[i] = ex + carry;
+    carry += total;
+  }
+}
+__global__ UBP_LB void k_scan_add(uint32_t* __restrict__ start, const uint32_t* __restrict__ bsum,
+                                  uint32_t ncodes) {
+  uint32_t add = bsum[blockIdx.x];
+  uint32_t base = blockIdx.x * 1024u + threadIdx.x * 4u;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (base + k < ncodes) start[base + k] += add;
+}
+
+// counting-sort scatter into arrival order within each cell
+__global__ UBP_LB void k_scatter(DevParams P, const uint32_t* __restrict__ key,
+                                 const uint32_t* __restrict__ arr, const uint32_t* __restrict__ start,
+                                 const uint32_t* __restrict__ orig, uint32_t* __restrict__ stage_src,
+                                 uint32_t* __restrict__ stage_orig) {
+  uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= P.n) return;
+  uint32_t d = start[key[s]] + arr[s];
+  stage_src[d] = s;
+  stage_orig[d] = orig[s];
+}
+
+// Stable order: rank each staged particle within its cell by original id
+// (the reference visits a cell's particles in ascending index,
+// neighbor.rs:112-121), then gather the particle state into that slot.
+__global__ UBP_LB void k_rank_gather(DevParams P, const uint32_t* __restrict__ stage_src,
+                                     const uint32_t* __restrict__ stage_orig,
+                                     const uint32_t* __restrict__ key, const uint32_t* __restrict__ start,
+                                     const float4* __restrict__ pos_t, const float4* __restrict__ vel_t,
+                                     const float* __restrict__ rho_src, const float4* __restrict__ wall_t,
+                                     const uint8_t* __restrict__ ft_src,
+                                     float4* __restrict__ pos, float4* __restrict__ vel,
+                                     float* __restrict__ rho_g, float4* __restrict__ wall,
+                                     uint8_t* __restrict__ ft_dst, uint32_t* __restrict__ orig_dst) {
+  uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= P.n) return;
+  uint32_t s = stage_src[d];
+  uint32_t c = key[s];
+  uint32_t b = start[c], e = start[c + 1u];
+  uint32_t me = stage_orig[d];
+  uint32_t rank = 0;
+  for (uint32_t k = b; k < e; ++k) rank += (stage_orig[k] < me) ? 1u : 0u;
+  uint32_t t = b + rank;
+  pos[t] = pos_t[s];
+  vel[t] = vel_t[s];
+  rho_g[t] = rho_src[s];
+  wall[t] = wall_t[s];
+  ft_dst[t] = ft_src[s];
+  orig_dst[t] = me;
+}
+
+// Walk the 27 cells around particle i in the reference order (dz, dy, dx;
+// neighbor.rs:147-183), slots ascending (= original id ascending), skip i,
+// accept dist^2 <= (2h)^2.  f(j, pos_j) is called for accepted neighbours.
+template <class F>
+__device__ __forceinline__ void walk27(const DevParams& P, uint32_t i, float4 pi,
+                                       const float4* __restrict__ pos,
+                                       const uint32_t* __restrict__ start, F&& f) {
+  uint32_t cx, cy, cz;
+  pos_to_cell(P, pi.x, pi.y, pi.z, cx, cy, cz);
+  for (int dz = -1; dz <= 1; ++dz) {
+    int nz = int(cz) + dz;
+    if (nz < 0 || nz >= int(P.dims[2])) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      int ny = int(cy) + dy;
+      if (ny < 0 || ny >= int(P.dims[1])) continue;
+      for (int dx = -1; dx <= 1; ++dx) {
+        int nx = int(cx) + dx;
+        if (nx < 0 || nx >= int(P.dims[0])) continue;
+        uint32_t code = morton(P, uint32_t(nx), uint32_t(ny), uint32_t(nz));
+        uint32_t b = start[code], e = start[code + 1u];
+        for (uint32_t s = b; s < e; ++s) {
+          if (s == i) continue;
+          float4 q = pos[s];
+          float ddx = pi.x - q.x;
+          float ddy = pi.y - q.y;
+          float ddz = pi.z - q.z;
+          float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+          if (d2 <= P.rsq) f(s, q);
+        }
+      }
+    }
+  }
+}
+
+// Neighbours of slot i in reference order: from the list when it fits,
+// otherwise by re-walking the cells (identical order).
+template <class F>
+__device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, float4 pi,
+                                              const float4* __restrict__ pos,
+                                              const uint32_t* __restrict__ start,
+                                              const uint32_t* __restrict__ nbr,
+                                              const uint32_t* __restrict__ nn, F&& f) {
+  uint32_t c = nn[i];
+  if (c <= P.K) {
+    const uint32_t* l = nbr + i;
+    for (uint32_t k = 0; k < c; ++k) {
+      uint32_t j = l[size_t(k) * P.n];
+      f(j, pos[j]);
+    }
+  } else {
+    walk27(P, i, pi, pos, start, f);
+  }
+}
+
+// Neighbour lists (cell walk once per step; reused by density, forces and the
+// next step's XSPH, which all see the same positions).
+__global__ UBP_LB void k_neigh(DevParams P, const float4* __restrict__ pos,
+                               const uint32_t* __restrict__ start, uint32_t* __restrict__ nbr,
+                               uint32_t* __restrict__ nn) {
+  uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  uint32_t i = blk * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  float4 pi = pos[i];
+  uint32_t c = 0;
+  uint32_t* l = nbr + i;
+  walk27(P, i, pi, pos, start, [&](uint32_t j, float4) {
+    if (c < P.K) l[size_t(c) * P.n] = j;
+    ++c;
+  });
+  nn[i] = c;
+}
+
+// Density summation + delta-SPH (old density) + SDF wall volume + Tait EOS.
+// sph.rs:112-160 (density), sph.rs:169-216 (diffusion), lib.rs:232-237,
+// eos.rs:39-47.
+__global__ UBP_LB void k_density(DevParams P, const float4* __restrict__ pos,
+                                 const float* __restrict__ rho_g, const float4* __restrict__ wall,
+                                 const uint8_t* __restrict__ ft, const uint32_t* __restrict__ start,
+                                 const uint32_t* __restrict__ nbr, const uint32_t* __restrict__ nn,
+                                 const float* __restrict__ dtp, float* __restrict__ rho_n,
+                                 float* __restrict__ prs, float* __restrict__ prho2) {
+  uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  uint32_t i = blk * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const float dt = dtp[0];
+  float4 pi = pos[i];
+  const float rho_i = rho_g[i];
+  Kahan rho, dif;
+  rho.init(pi.w * P.norm_h3);  // m_i W(0, h)
+  dif.init(0.0f);
+  for_neighbors(P, i, pi, pos, start, nbr, nn, [&](uint32_t j, float4 q) {
+    float dx = pi.x - q.x;
+    float dy = pi.y - q.y;
+    float dz = pi.z - q.z;
+    float dist_sq = dx * dx + dy * dy + dz * dz;
+    float r = sqrtf(dist_sq);
+    float qq = r / P.h;
+    float gx, gy, gz;
+    wgrad_q(dx, dy, dz, r, qq, P, gx, gy, gz);
+    float rho_j = rho_g[j];
+    float rho_jd = rho_j;
+    if (P.diff_floor > 0.0f && !(rho_jd >= P.diff_floor)) rho_jd = P.diff_floor;
+    float v_j = q.w / rho_jd;
+    float r_dot_grad = dx * gx + dy * gy + dz * gz;
+    float density_diff = rho_i - rho_j;
+    dif.add(v_j * density_diff * 2.0f * r_dot_grad / (dist_sq + P.eta_sq));
+    rho.add(q.w * wendland_q(qq, P));
+  });
+  const int f = ft[i];
+  float4 w = wall[i];
+  if (w.w != __builtin_inff()) {
+    float s = w.w / P.h;
+    if (s < 2.0f) rho.add(P.rho0[f] * wall_omega(s));
+  }
+  float diffusion = P.diff_coef * dif.s;
+  float rn = rho.s + diffusion * dt;
+  float p = tait(rn, f, P);
+  rho_n[i] = rn;
+  prs[i] = p;
+  prho2[i] = p / (rn * rn);
+}
+
+// Pressure (sph.rs:322-417) + Monaghan viscosity (sph.rs:432-492) + gravity
+// (sph.rs:502-508) + SDF wall pressure and repulsion.
+__global__ UBP_LB void k_forces(DevParams P, const float4* __restrict__ pos,
+                                const float4* __restrict__ vel, const float* __restrict__ rho_n,
+                                const float* __restrict__ prs, const float* __restrict__ prho2,
+                                const float4* __restrict__ wall, const uint8_t* __restrict__ ft,
+                                const uint32_t* __restrict__ start, const uint32_t* __restrict__ nbr,
+                                const uint32_t* __restrict__ nn, float4* __restrict__ acc) {
+  uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  uint32_t i = blk * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  float4 pi = pos[i];
+  float4 vi = vel[i];
+  const float rho_i = rho_n[i];
+  const float pi_over = prho2[i];
+  const float nmi = -pi.w;
+  Kahan fx, fy, fz, vx, vy, vz;
+  fx.init(0.0f); fy.init(0.0f); fz.init(0.0f);
+  vx.init(0.0f); vy.init(0.0f); vz.init(0.0f);
+  for_neighbors(P, i, pi, pos, start, nbr, nn, [&](uint32_t j, float4 q) {
+    float dx = pi.x - q.x;
+    float dy = pi.y - q.y;
+    float dz = pi.z - q.z;
+    float r = sqrtf(dx * dx + dy * dy + dz * dz);
+    float qq = r / P.h;
+    float gx, gy, gz;
+    wgrad_q(dx, dy, dz, r, qq, P, gx, gy, gz);
+    float factor = nmi * q.w * (pi_over + prho2[j]);
+    fx.add(factor * gx); fy.add(factor * gy); fz.add(factor * gz);
+    float4 vj = vel[j];
+    float dvx = vi.x - vj.x;
+    float dvy = vi.y - vj.y;
+    float dvz = vi.z - vj.z;
+    float vr = dvx * dx + dvy * dy + dvz * dz;
+    if (vr < 0.0f) {
+      float r_sq = r * r;
+      float mu = P.h * vr / (r_sq + P.eta_sq);
+      float rho_avg = 0.5f * (rho_i + rho_n[j]);
+      float pij = (P.nac * mu + P.beta * mu * mu) / rho_avg;
+      float vf = -q.w * pij;
+      vx.add(vf * gx); vy.add(vf * gy); vz.add(vf * gz);
+    }
+  });
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  ax += fx.s / pi.w; ay += fy.s / pi.w; az += fz.s / pi.w;
+  ax += vx.s; ay += vy.s; az += vz.s;
+  ax += P.g[0]; ay += P.g[1]; az += P.g[2];
+  float4 w = wall[i];
+  if (w.w != __builtin_inff()) {
+    float s = w.w / P.h;
+    if (s < 2.0f) {
+      float pr = prs[i];
+      float pic = (pr > 0.0f) ? pr : 0.0f;
+      int f = ft[i];
+      float coef = 2.0f * (pic / (rho_i * rho_i)) * P.rho0[f] * wall_gamma(s) / P.h;
+      ax += coef * w.x; ay += coef * w.y; az += coef * w.z;
+      if (w.w < P.r0) {
+        float dd = (w.w > 0.0f) ? w.w : 0.0f;
+        float sr0 = 1.0f - dd / P.r0;
+        float fm = P.d_rep * sr0 * sr0 / P.r0;
+        ax += fm * w.x; ay += fm * w.y; az += fm * w.z;
+      }
+    }
+  }
+  acc[i] = make_float4(ax, ay, az, 0.0f);
+}
+
+// v += a dt/2 (lib.rs:301-305)
+__global__ UBP_LB void k_kick(DevParams P, float4* __restrict__ vel, const float4* __restrict__ acc,
+                              const float* __restrict__ dtp) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const float half_dt = 0.5f * dtp[0];
+  float4 v = vel[i];
+  float4 a = acc[i];
+  v.x += a.x * half_dt;
+  v.y += a.y * half_dt;
+  v.z += a.z * half_dt;
+  vel[i] = v;
+}
+
+// Second half-kick (lib.rs:330-334) fused with the CFL reduction of
+// compute_timestep (sph.rs:591-628) for the next step: max(|v| + c_s) and
+// max |a| as float bits (non-negative, so integer max == float max); the last
+// workgroup turns them into dt and re-arms the reduction slots.
+// red[0] = max signal bits, red[1] = max accel bits, red[2] = arrival counter.
+__global__ UBP_LB void k_kick2_cfl(DevParams P, float4* __restrict__ vel, const float4* __restrict__ acc,
+                                   float* __restrict__ dtp, uint32_t* __restrict__ red, int do_kick,
+                                   int do_cfl) {
+  __shared__ uint32_t smax[2][4];
+  __shared__ int last;
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const float half_dt = 0.5f * dtp[0];
+  float sig = 0.0f, am = 0.0f;
+  if (i < P.n) {
+    float4 v = vel[i];
+    float4 a = acc[i];
+    if (do_kick) {
+      v.x += a.x * half_dt;
+      v.y += a.y * half_dt;
+      v.z += a.z * half_dt;
+      vel[i] = v;
+    }
+    float vm = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z);
+    float sg = vm + P.cs;
+    if (sg > sig) sig = sg;  // NaN never wins (reference `>` test)
+    float an = sqrtf(a.x * a.x + a.y * a.y + a.z * a.z);
+    if (an > am) am = an;
+  }
+  if (!do_cfl) return;
+  uint32_t bs = __float_as_uint(sig), ba = __float_as_uint(am);
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t t0 = __shfl_xor(bs, o, 64);
+    uint32_t t1 = __shfl_xor(ba, o, 64);
+    bs = bs > t0 ? bs : t0;
+    ba = ba > t1 ? ba : t1;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) { smax[0][wv] = bs; smax[1][wv] = ba; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t m0 = 0, m1 = 0;
+    for (int k = 0; k < int(blockDim.x >> 6); ++k) {
+      m0 = m0 > smax[0][k] ? m0 : smax[0][k];
+      m1 = m1 > smax[1][k] ? m1 : smax[1][k];
+    }
+    atomicMax(&red[0], m0);
+    atomicMax(&red[1], m1);
+    __threadfence();
+    uint32_t prev = atomicAdd(&red[2], 1u);
+    last = (prev == gridDim.x - 1u);
+  }
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    __threadfence();
+    uint32_t m0 = atomicAdd(&red[0], 0u);
+    uint32_t m1 = atomicAdd(&red[1], 0u);
+    float max_signal = __uint_as_float(m0);
+    if (!(max_signal > P.cs)) max_signal = P.cs;  // starts at c_s
+    float max_acc = __uint_as_float(m1);
+    float dt_cfl = P.cfl * P.h / max_signal;
+    float dt_force = (max_acc > 1.0e-12f) ? 0.25f * sqrtf(P.h / max_acc) : 0.01f;
+    float dt = (dt_force < dt_cfl) ? dt_force : dt_cfl;
+    if (dt < 1.0e-8f) dt = 1.0e-8f;
+    if (dt > 0.01f) dt = 0.01f;
+    dtp[0] = dt;
+    red[0] = 0u;
+    red[1] = 0u;
+    red[2] = 0u;
+  }
+}
+
+// XSPH (sph.rs:236-280) on the post-kick velocities, drift (lib.rs:313-317),
+// SDF wall projection, domain clamp (boundary.rs:239-291), then the wall
+// record, cell key and histogram of the new position (neighbor.rs:89-101).
+__global__ UBP_LB void k_xsph_drift(DevParams P, const float4* __restrict__ pos,
+                                    const float4* __restrict__ vel, const float* __restrict__ rho_n,
+                                    const uint32_t* __restrict__ start, const uint32_t* __restrict__ nbr,
+                                    const uint32_t* __restrict__ nn, const float* __restrict__ dtp,
+                                    float4* __restrict__ pos_t, float4* __restrict__ vel_t,
+                                    float4* __restrict__ wall_t, uint32_t* __restrict__ key,
+                                    uint32_t* __restrict__ arr, uint32_t* __restrict__ cnt) {
+  uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  uint32_t i = blk * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const float dt = dtp[0];
+  float4 pi = pos[i];
+  float4 vi = vel[i];
+  const float rho_i = rho_n[i];
+  float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+  for_neighbors(P, i, pi, pos, start, nbr, nn, [&](uint32_t j, float4 q) {
+    float dx = pi.x - q.x;
+    float dy = pi.y - q.y;
+    float dz = pi.z - q.z;
+    float r = sqrtf(dx * dx + dy * dy + dz * dz);
+    float w = wendland_q(r / P.h, P);
+    float rho_avg = 0.5f * (rho_i + rho_n[j]);
+    float den = (rho_avg >= 1.0f) ? rho_avg : 1.0f;
+    float factor = q.w / den * w;
+    float4 vj = vel[j];
+    cx += factor * (vj.x - vi.x);
+    cy += factor * (vj.y - vi.y);
+    cz += factor * (vj.z - vi.z);
+  });
+  float dvx = 0.5f * cx, dvy = 0.5f * cy, dvz = 0.5f * cz;
+  float x = pi.x + (vi.x + dvx) * dt;
+  float y = pi.y + (vi.y + dvy) * dt;
+  float z = pi.z + (vi.z + dvz) * dt;
+  float vx = vi.x, vy = vi.y, vz = vi.z;
+  if (P.sdf_on) {
+    float4 w = wall_at(P, x, y, z);
+    if (w.w != __builtin_inff() && w.w < 0.0f) {
+      float d = w.w;
+      x -= d * w.x; y -= d * w.y; z -= d * w.z;
+      float vn = vx * w.x + vy * w.y + vz * w.z;
+      if (vn < 0.0f) {
+        float k = (1.0f + 0.2f) * vn;
+        vx -= k * w.x; vy -= k * w.y; vz -= k * w.z;
+      }
+    }
+  }
+  const float rest = 0.2f;
+  if (x < P.dmin[0]) { x = P.dmin[0]; if (vx < 0.0f) vx = -rest * vx; }
+  if (x > P.dmax[0]) { x = P.dmax[0]; if (vx > 0.0f) vx = -rest * vx; }
+  if (y < P.dmin[1]) { y = P.dmin[1]; if (vy < 0.0f) vy = -rest * vy; }
+  if (y > P.dmax[1]) { y = P.dmax[1]; if (vy > 0.0f) vy = -rest * vy; }
+  if (z < P.dmin[2]) { z = P.dmin[2]; if (vz < 0.0f) vz = -rest * vz; }
+  if (z > P.dmax[2]) { z = P.dmax[2]; if (vz > 0.0f) vz = -rest * vz; }
+  pos_t[i] = make_float4(x, y, z, pi.w);
+  vel_t[i] = make_float4(vx, vy, vz, 0.0f);
+  wall_t[i] = wall_at(P, x, y, z);
+  uint32_t ccx, ccy, ccz;
+  pos_to_cell(P, x, y, z, ccx, ccy, ccz);
+  uint32_t code = morton(P, ccx, ccy, ccz);
+  key[i] = code;
+  arr[i] = atomicAdd(&cnt[code], 1u);
+}
+
+// ---- diagnostics ----
+// Cell key of every slot's current position, scattered to original order.
+__global__ UBP_LB void k_debug_keys(DevParams P, const float4* __restrict__ pos,
+                                    const uint32_t* __restrict__ orig, uint32_t* __restrict__ out) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  float4 p = pos[i];
+  uint32_t cx, cy, cz;
+  pos_to_cell(P, p.x, p.y, p.z, cx, cy, cz);
+  out[orig[i]] = morton(P, cx, cy, cz);
+}
+
+// ErrorMetrics partials (lib.rs:341-377): per block max |rho-rho0|/rho0 (f32)
+// and FP64 sums of energy and mass in a fixed tree order.
+__global__ UBP_LB void k_metrics(DevParams P, const float4* __restrict__ pos,
+                                 const float4* __restrict__ vel, const float* __restrict__ rho,
+                                 const uint8_t* __restrict__ ft, float* __restrict__ pmax,
+                                 double* __restrict__ psum) {
+  __shared__ double se[256], sm[256];
+  __shared__ float sv[256];
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0.0, m = 0.0;
+  float var = 0.0f;
+  if (i < P.n) {
+    float4 p = pos[i];
+    float4 v = vel[i];
+    int f = ft[i];
+    float r0 = P.rho0[f];
+    var = fabsf(rho[i] - r0) / r0;
+    double md = double(p.w);
+    double vx = v.x, vy = v.y, vz = v.z;
+    e = 0.5 * md * (vx * vx + vy * vy + vz * vz);
+    e -= md * (double(P.g[0]) * double(p.x) + double(P.g[1]) * double(p.y) + double(P.g[2]) * double(p.z));
+    m = md;
+  }
+  se[threadIdx.x] = e; sm[threadIdx.x] = m; sv[threadIdx.x] = var;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (int(threadIdx.x) < s) {
+      se[threadIdx.x] += se[threadIdx.x + s];
+      sm[threadIdx.x] += sm[threadIdx.x + s];
+      sv[threadIdx.x] = fmaxf(sv[threadIdx.x], sv[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    pmax[blockIdx.x] = sv[0];
+    psum[2 * blockIdx.x] = se[0];
+    psum[2 * blockIdx.x + 1] = sm[0];
+  }
+}
+
+}  // namespace ubp
