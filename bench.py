@@ -1,0 +1,170 @@
+#!/usr/bin/env python3
+"""Throughput of the WCSPH step on the 1,048,576-particle 3D dam break.
+
+One "step" is one kick-drift-kick WCSPH step (binning, density + delta-SPH +
+EOS, forces, XSPH drift, SDF walls, CFL dt on the device) over the whole
+particle set.  Inputs are resident in HBM before the timed region; dt is
+computed on the device each step.
+
+Prints ONE JSON line (rank 0) with the metric, a ``roofline`` object for the
+dominant kernel (algorithmic bytes per launch / its average HIP-event time)
+and a ``cpu_baseline`` object (the CPU oracle on the host cores, bounded
+sample).  Single GPU only.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import ultraballpit_amd as ubp  # noqa: E402
+from ultraballpit_amd import scenes  # noqa: E402
+
+METRIC = "particle-steps/s @1M dam break, %HBM roofline; surrogate/plain speedup @≤1% err"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic (compulsory) bytes per particle per launch, FP32 SoA fields each
+# read or written once per pass that needs them; neighbour re-reads are not
+# counted.  Sum over the step = 280 B per particle-step.
+ALG_BYTES = {
+    "k_xsph_drift": 44 + 72,   # XSPH (R pos, vel, rho, m; W dv) + fused KDK/SDF/CFL drift
+    "k_scan": 0,
+    "k_scatter": 8,            # R key, W perm
+    "k_rank_gather": 64,       # SoA gather R+W pos, vel, rho, m
+    "k_neigh": 16,             # R pos; W key (cell key pass)
+    "k_density": 28,           # R pos 12, rho_old 4, m 4; W rho 4, P 4
+    "k_forces": 48,            # R pos 12, vel 12, rho 4, P 4, m 4; W acc 12
+    "k_kick": 0, "k_kick2_cfl": 0,
+}
+STEP_ALG_BYTES = 280
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles(kernel: str):
+    """HBM bytes per dispatch of `kernel` from the committed rocprof PMC
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_dam_break_1m_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"^traffic_bytes_per_dispatch:\s*([0-9.eE+]+)", open(path).read(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(scene, budget_s: float):
+    """The CPU oracle (engine arithmetic, OpenMP over all allotted threads) on
+    a bounded number of steps of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle  # noqa: E402  (checker only: the CPU baseline leg)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
+    threads = pyoracle.set_threads(threads)
+    o = pyoracle.Oracle(scene, precision=32, flags=pyoracle.ENGINE)
+    n = len(scene["x"])
+    t0 = time.perf_counter()
+    o.step_auto(1)  # includes the bootstrap force pass
+    t1 = time.perf_counter()
+    first = t1 - t0
+    steps = max(1, min(20, int(budget_s / max(first, 1e-3))))
+    t1 = time.perf_counter()
+    o.step_auto(steps)
+    t2 = time.perf_counter()
+    rate = n * steps / (t2 - t1)
+    return {"value": rate, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} steady steps of the {n}-particle dam break after 1 bootstrap step, "
+                      f"C++ oracle (engine arithmetic), OpenMP {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--particles", type=int, default=1 << 20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-graph", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the dam break would shard)", file=sys.stderr)
+        sys.exit(2)
+
+    log(f"[bench] building dam-break scene ({args.particles} particles)")
+    sc = scenes.dam_break(n_keep=args.particles)
+    n = len(sc["x"])
+    eng = ubp.HipKernel.from_scene(sc)
+    dims, _ = eng.grid_info()
+    log(f"[bench] warmup {args.warmup} steps")
+    eng.step_n(args.warmup, use_graph=not args.no_graph)
+    eng.sync()
+
+    # timed region A: graph-replayed device-dt steps (the headline value)
+    log(f"[bench] timed {args.steps} steps (graph)")
+    eng.sync()
+    t0 = time.perf_counter()
+    eng.step_n(args.steps, use_graph=not args.no_graph)
+    eng.sync()
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    ms_per_step = wall * 1e3 / args.steps
+    value = n * args.steps / wall
+
+    # timed region B: the same steps with HIP events around every kernel on
+    # the engine's stream (eager launches) for the per-kernel roofline
+    log(f"[bench] per-kernel event timing over {args.steps} steps")
+    t2 = time.perf_counter()
+    kms = eng.step_n_profiled(args.steps)
+    t3 = time.perf_counter()
+    per_kernel_us = {k: v * 1e3 / args.steps for k, v in kms.items()}
+    dom = max(per_kernel_us, key=per_kernel_us.get)
+    avg_s = per_kernel_us[dom] * 1e-6
+    alg = ALG_BYTES.get(dom, 0) * n
+    achieved = alg / avg_s / 1e9 if avg_s > 0 else 0.0
+    traffic = traffic_from_profiles(dom)
+    s = eng.particles()
+    finite = bool(all(np.isfinite(s[k]).all() for k in ("x", "y", "z", "vx", "vy", "vz", "density")))
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "3D dam break WCSPH, 1,048,576 particles, 1.0x0.25x0.6 m tank, "
+                               "h = 1.3 dx, c0 = 30, alpha = 0.02, CFL 0.25, box SDF 256x64x160",
+                   "particles": n, "grid": [int(d) for d in dims], "parallelism": "single GPU",
+                   "dt_on_device": True, "graph": not args.no_graph},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "alg_bytes_per_launch": alg, "avg_launch_us": per_kernel_us[dom]},
+        "step_hbm_frac": STEP_ALG_BYTES * value / (HBM_PEAK_GBS * 1e9),
+        "kernel_us_per_step": {k: round(v, 2) for k, v in per_kernel_us.items()},
+        "eager_ms_per_step": (t3 - t2) * 1e3 / args.steps,
+        "state_finite": finite,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        log("[bench] CPU baseline (oracle) on the host cores")
+        out["cpu_baseline"] = cpu_baseline(sc, args.cpu_budget)
+    eng.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -134,6 +134,22 @@ int ubp_get_dt(ubp_engine *e, float *dt);
 /* GpuKernel::step_profiled (backend/crates/kernel/src/gpu/mod.rs:1043-1278). */
 int ubp_step_profiled(ubp_engine *e, float dt, ubp_step_profile *out);
 
+/* Per-kernel device time of n device-dt steps (successor of the per-pass
+ * timestamp table of step_profiled, gpu/mod.rs:1043-1278 and
+ * benches/gpu_throughput.rs:136-161).  HIP events are recorded on the
+ * handle's stream around every launch; kernel_ms[k] receives the summed
+ * milliseconds of kernel k over the n steps, in the order:
+ *   0 kick, 1 xsph_drift, 2 scan (3 launches), 3 scatter, 4 rank_gather,
+ *   5 neigh, 6 density, 7 forces, 8 kick2_cfl.
+ * Synchronous. */
+#define UBP_NKERN 9
+int ubp_step_n_profiled(ubp_engine *e, int nsteps, double *kernel_ms);
+
+/* Measurement utility, no reference counterpart: `reps` launches of a plain
+ * float4 copy kernel (k_calib_copy) over `bytes` bytes on the current device;
+ * returns the mean kernel time in *ms.  Used to scale HBM counters. */
+int ubp_calibration_copy(size_t bytes, int reps, double *ms);
+
 /* SimulationKernel::particles (lib.rs:63; GPU readback gpu/buffers.rs:
  * 360-477), into caller buffers in ORIGINAL particle order.  Synchronous. */
 int ubp_read_particles(ubp_engine *e, const ubp_particles_out *out);

@@ -1,0 +1,137 @@
+"""CPU-side checks of the product boundary: the C-ABI library loads and
+exports every symbol include/ubp.h declares, fails cleanly without a GPU, and
+the host-side mirror of the reference API behaves like the reference
+(orchestrator/src/lib.rs:142-266, kernel/src/sph.rs:532-628)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import ultraballpit_amd as ubp
+from ultraballpit_amd import scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "ubp.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(ubp_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_header_symbol():
+    syms = header_symbols()
+    assert len(syms) >= 20
+    L = C.CDLL(ubp.library_path())
+    for s in syms:
+        assert hasattr(L, s), s
+    assert set(syms) == set(ubp.EXPORTS)
+    # dynamic symbol table agrees (extern "C", unmangled)
+    out = subprocess.run(["nm", "-D", "--defined-only", ubp.library_path()], capture_output=True,
+                         text=True).stdout
+    for s in syms:
+        assert re.search(rf"\bT {s}$", out, re.M), s
+
+
+def test_library_targets_gfx950_only(tmp_path):
+    fb = tmp_path / "ubp.fatbin"
+    r = subprocess.run(["/opt/rocm/llvm/bin/llvm-objcopy", f"--dump-section=.hip_fatbin={fb}",
+                        ubp.library_path(), str(tmp_path / "stripped.so")], capture_output=True)
+    if r.returncode != 0:
+        pytest.skip("llvm-objcopy unavailable")
+    out = subprocess.run(["/opt/rocm/llvm/bin/clang-offload-bundler", "--list", "--type=o",
+                          f"--input={fb}"], capture_output=True, text=True)
+    targets = [t for t in out.stdout.split() if "amdgcn" in t]
+    assert targets == ["hipv4-amdgcn-amd-amdhsa--gfx950"]
+
+
+@pytest.mark.skipif(ubp.gpu_available(), reason="CPU-only check")
+def test_create_fails_cleanly_without_device():
+    sc = scenes.water_box(n_side=4, dx=1e-3, box=(0.004, 0.004, 0.005), sdf_nodes=16)
+    assert ubp.gpu_available() is False
+    with pytest.raises(ubp.GpuInitError, match="device"):
+        ubp.HipKernel.from_scene(sc)
+
+
+def test_auto_tune_speed_of_sound():
+    # sph.rs:532-572: min(max(10 sqrt(2 g H), 1), c_cfg); no gravity -> max(c_cfg, 1)
+    c = ubp.auto_tune_speed_of_sound((0, -9.81, 0), (0, 0, 0), (0.01, 0.01, 0.01), 50.0)
+    assert c == pytest.approx(10 * np.sqrt(2 * 9.81 * 0.01), rel=1e-6)
+    assert ubp.auto_tune_speed_of_sound((0, -9.81, 0), (0, 0, 0), (1, 1, 1), 5.0) == 5.0
+    assert ubp.auto_tune_speed_of_sound((0, 0, 0), (0, 0, 0), (1, 1, 1), 0.5) == 1.0
+    # height is the extent projected on |g|
+    c = ubp.auto_tune_speed_of_sound((0, 0, -9.81), (0, 0, 0), (1.0, 0.25, 0.6), 1e9)
+    assert c == pytest.approx(10 * np.sqrt(2 * 9.81 * 0.6), rel=1e-6)
+
+
+def test_compute_timestep_matches_oracle(oracle):
+    rng = np.random.default_rng(5)
+    sc = scenes.dam_break_small(lat=(6, 6, 12))
+    n = len(sc["x"])
+    sc["vx"] = rng.normal(0, 2, n).astype(np.float32)
+    sc["vz"] = rng.normal(0, 1, n).astype(np.float32)
+    sc["az"] = rng.normal(0, 50, n).astype(np.float32)
+    o = oracle.Oracle(sc)
+    got = ubp.compute_timestep(sc | {"ax": np.zeros(n), "ay": np.zeros(n), "vy": np.zeros(n)},
+                               sc["h"], sc["speed_of_sound"], sc["cfl_number"])
+    assert np.float32(got) == np.float32(o.timestep())
+
+
+class _FakeCpu:
+    def __init__(self, *args):
+        self.args = args
+
+
+def test_create_kernel_selection():
+    sc = scenes.water_box(n_side=4, dx=1e-3, box=(0.004, 0.004, 0.005), sdf_nodes=16)
+    args = (sc, None, sc["h"], sc["gravity"], 50.0, 0.25, 0.0, sc["domain_min"], sc["domain_max"])
+    # CPU: always the caller's CPU backend, with the auto-tuned c_s
+    k = ubp.create_kernel(ubp.BackendType.CPU, *args, cpu_factory=_FakeCpu)
+    assert isinstance(k, _FakeCpu)
+    assert k.args[4] == pytest.approx(10 * np.sqrt(2 * 9.81 * 0.005), rel=1e-6)
+    if not ubp.gpu_available():
+        # AUTO falls back (orchestrator/src/lib.rs:200-249); GPU raises
+        assert isinstance(ubp.create_kernel(ubp.BackendType.AUTO, *args, cpu_factory=_FakeCpu), _FakeCpu)
+        with pytest.raises(ubp.GpuInitError):
+            ubp.create_kernel(ubp.BackendType.GPU, *args)
+        with pytest.raises(ubp.GpuInitError):
+            ubp.create_kernel(ubp.BackendType.AUTO, *args)
+
+
+def test_product_path_does_not_reach_the_oracle():
+    # the package never imports or loads the oracle
+    pkg = os.path.join(ROOT, "ultraballpit_amd")
+    for dp, _, fs in os.walk(pkg):
+        for f in fs:
+            if f.endswith((".py", ".hip", ".hpp", ".cpp", ".h")):
+                src = open(os.path.join(dp, f)).read()
+                assert "pyoracle" not in src and "ubp_oracle" not in src and "orc_" not in src, f
+
+
+def test_scenes_shapes_and_determinism():
+    sc = scenes.dam_break()
+    assert len(sc["x"]) == 1 << 20
+    dx = sc["dx"]
+    assert abs(float(sc["h"]) - 1.3 * dx) < 1e-9
+    # jitter is within +-0.05 dx of the lattice
+    lat = scenes.lattice(81, 81, 162, dx, n_keep=1 << 20)
+    for a, k in enumerate("xyz"):
+        assert np.max(np.abs(sc[k] - lat[:, a])) <= 0.05 * dx + 1e-7  # + FP32 rounding
+    sc2 = scenes.dam_break()
+    assert np.array_equal(sc["x"], sc2["x"])
+    assert tuple(sc["sdf"]["dims"]) == (256, 64, 160)
+    wb = scenes.water_box()
+    assert len(wb["x"]) == 8000 and abs(float(wb["h"]) - 0.65e-3) < 1e-9
+    u = scenes.uniform(1, 2, 1000)
+    assert 0.0 <= u.min() and u.max() < 1.0 and abs(u.mean() - 0.5) < 0.05
+
+
+def test_sweep_scenes_counts():
+    sw = scenes.sweep_scenes(n_scenes=16, n_min=2000, n_max=20000)
+    ns = [len(s["x"]) for s in sw]
+    assert min(ns) >= 1000 and max(ns) <= 40000
+    alphas = [float(s["alpha"]) for s in sw]
+    assert all(0.005 <= a <= 0.05 for a in alphas)

@@ -81,7 +81,12 @@ EXPORTS = ("ubp_create", "ubp_destroy", "ubp_last_error", "ubp_gpu_available",
            "ubp_set_viscosity_coeffs", "ubp_set_sdf", "ubp_set_tuning", "ubp_step",
            "ubp_step_async", "ubp_sync", "ubp_step_n", "ubp_get_dt", "ubp_step_profiled",
            "ubp_read_particles", "ubp_particle_count", "ubp_error_metrics",
-           "ubp_save_checkpoint", "ubp_restore_checkpoint", "ubp_debug_bins", "ubp_grid_info")
+           "ubp_save_checkpoint", "ubp_restore_checkpoint", "ubp_debug_bins", "ubp_grid_info",
+           "ubp_step_n_profiled", "ubp_calibration_copy")
+
+#: kernel order of ubp_step_n_profiled
+KERNELS = ("k_kick", "k_xsph_drift", "k_scan", "k_scatter", "k_rank_gather", "k_neigh",
+           "k_density", "k_forces", "k_kick2_cfl")
 
 
 def library_path() -> str:
@@ -124,6 +129,8 @@ def lib():
     L.ubp_restore_checkpoint.argtypes = [C.c_void_p]
     L.ubp_debug_bins.argtypes = [C.c_void_p, u32p, u32p, u32p]
     L.ubp_grid_info.argtypes = [C.c_void_p, u32p, u32p]
+    L.ubp_step_n_profiled.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
+    L.ubp_calibration_copy.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_double)]
     for name in EXPORTS:
         getattr(L, name)
     _lib = L
@@ -313,6 +320,13 @@ class HipKernel:
         _check(lib().ubp_step_profiled(self._h, float(dt), C.byref(p)), "step_profiled")
         return {k: int(getattr(p, k)) for k, _ in _Profile._fields_}
 
+    def step_n_profiled(self, nsteps: int) -> dict:
+        """n device-dt steps; summed per-kernel milliseconds (HIP events)."""
+        out = np.zeros(len(KERNELS), np.float64)
+        _check(lib().ubp_step_n_profiled(self._h, int(nsteps), out.ctypes.data_as(C.POINTER(C.c_double))),
+               "step_n_profiled")
+        return dict(zip(KERNELS, out.tolist()))
+
     def debug_bins(self):
         n = self._n
         keys = np.empty(n, np.uint32)
@@ -326,6 +340,13 @@ class HipKernel:
         b = np.zeros(3, np.uint32)
         _check(lib().ubp_grid_info(self._h, _u32p(d), _u32p(b)), "grid_info")
         return d, b
+
+
+def calibration_copy(nbytes: int, reps: int = 20) -> float:
+    """Mean time (ms) of a plain float4 copy kernel over nbytes."""
+    ms = C.c_double(0.0)
+    _check(lib().ubp_calibration_copy(int(nbytes), int(reps), C.byref(ms)), "calibration_copy")
+    return float(ms.value)
 
 
 # ---- host-side helpers of the reference API ------------------------------

@@ -435,6 +435,62 @@ struct Engine {
 
   void sync() { HIPCHK(hipStreamSynchronize(st)); }
 
+  // device-dt steps with HIP events around every launch (eager launches)
+  void step_n_profiled(int nsteps, double* kernel_ms) {
+    for (int k = 0; k < UBP_NKERN; ++k) kernel_ms[k] = 0.0;
+    if (n == 0 || nsteps <= 0) return;
+    stepped = true;
+    ensure_dt();
+    if (needs_init) { bootstrap(); step_body(1); --nsteps; }
+    if (nsteps <= 0) { sync(); return; }
+    const int per = UBP_NKERN + 1;
+    std::vector<hipEvent_t> ev(size_t(nsteps) * per, nullptr);
+    try {
+      for (auto& x : ev) HIPCHK(hipEventCreate(&x));
+      const uint32_t g = grid_of(n);
+      for (int s = 0; s < nsteps; ++s) {
+        hipEvent_t* E = &ev[size_t(s) * per];
+        HIPCHK(hipEventRecord(E[0], st));
+        launch(k_kick, g, P, vel, acc, dtp);
+        HIPCHK(hipEventRecord(E[1], st));
+        launch(k_xsph_drift, g, P, pos, vel, rho_n, start, nbr, nn, dtp, pos_t, vel_t, wall_t, key, arr, cnt);
+        HIPCHK(hipEventRecord(E[2], st));
+        hipLaunchKernelGGL(k_scan_local, dim3(nblk_scan), dim3(256), 0, st, cnt, start, bsum, P.ncodes);
+        hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(256), 0, st, bsum, nblk_scan);
+        hipLaunchKernelGGL(k_scan_add, dim3(nblk_scan), dim3(256), 0, st, start, bsum, P.ncodes);
+        LAUNCHCHK("scan");
+        HIPCHK(hipEventRecord(E[3], st));
+        launch(k_scatter, g, P, key, arr, start, orig[pp], stage_src, stage_orig);
+        HIPCHK(hipEventRecord(E[4], st));
+        launch(k_rank_gather, g, P, stage_src, stage_orig, key, start, pos_t, vel_t, rho_n, wall_t, ft[pp],
+               pos, vel, rho_g, wall, ft[pp ^ 1], orig[pp ^ 1]);
+        pp ^= 1;
+        HIPCHK(hipEventRecord(E[5], st));
+        launch(k_neigh, g, P, pos, start, nbr, nn);
+        HIPCHK(hipEventRecord(E[6], st));
+        launch(k_density, g, P, pos, rho_g, wall, ft[pp], start, nbr, nn, dtp, rho_n, prs, prho2);
+        HIPCHK(hipEventRecord(E[7], st));
+        launch(k_forces, g, P, pos, vel, rho_n, prs, prho2, wall, ft[pp], start, nbr, nn, acc);
+        HIPCHK(hipEventRecord(E[8], st));
+        launch(k_kick2_cfl, g, P, vel, acc, dtp, red, 1, 1);
+        HIPCHK(hipEventRecord(E[9], st));
+      }
+      sync();
+      for (int s = 0; s < nsteps; ++s) {
+        hipEvent_t* E = &ev[size_t(s) * per];
+        for (int k = 0; k < UBP_NKERN; ++k) {
+          float ms = 0.0f;
+          HIPCHK(hipEventElapsedTime(&ms, E[k], E[k + 1]));
+          kernel_ms[k] += double(ms);
+        }
+      }
+    } catch (...) {
+      for (auto& x : ev) if (x) (void)hipEventDestroy(x);
+      throw;
+    }
+    for (auto& x : ev) (void)hipEventDestroy(x);
+  }
+
   void ensure_stage(size_t bytes) {
     if (h_stage_bytes >= bytes) return;
     if (h_stage) (void)hipHostFree(h_stage);
@@ -779,6 +835,56 @@ int ubp_step_profiled(ubp_engine* e, float dt, ubp_step_profile* out) {
   out->boundary_pressure_us = 0;
   out->readback_us = uint64_t(std::chrono::duration<double, std::micro>(t2 - t1).count());
   out->total_us = uint64_t(std::chrono::duration<double, std::micro>(t2 - t0).count());
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_step_n_profiled(ubp_engine* e, int nsteps, double* kernel_ms) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  if (!kernel_ms) throw Error(UBP_EINVAL, "null kernel_ms");
+  if (nsteps < 0) throw Error(UBP_EINVAL, "nsteps < 0");
+  e->eng.step_n_profiled(nsteps, kernel_ms);
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_calibration_copy(size_t bytes, int reps, double* ms) {
+  UBP_API_BEGIN
+  if (!ms || reps <= 0 || bytes < 16) throw Error(UBP_EINVAL, "bad calibration arguments");
+  size_t n4 = bytes / 16;
+  float4 *src = nullptr, *dst = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t a = nullptr, b = nullptr;
+  try {
+    ubp::dalloc(&src, n4);
+    ubp::dalloc(&dst, n4);
+    HIPCHK(hipMemset(src, 0, n4 * 16));
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    uint32_t grid = uint32_t(std::min<size_t>((n4 + 255) / 256, 256 * 1024));
+    hipLaunchKernelGGL(ubp::k_calib_copy, dim3(grid), dim3(256), 0, s, (const float4*)src, dst, n4);
+    HIPCHK(hipEventRecord(a, s));
+    for (int r = 0; r < reps; ++r)
+      hipLaunchKernelGGL(ubp::k_calib_copy, dim3(grid), dim3(256), 0, s, (const float4*)src, dst, n4);
+    HIPCHK(hipEventRecord(b, s));
+    HIPCHK(hipStreamSynchronize(s));
+    float t = 0.0f;
+    HIPCHK(hipEventElapsedTime(&t, a, b));
+    *ms = double(t) / reps;
+  } catch (...) {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    if (s) (void)hipStreamDestroy(s);
+    ubp::dfree(src); ubp::dfree(dst);
+    throw;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  (void)hipStreamDestroy(s);
+  ubp::dfree(src);
+  ubp::dfree(dst);
   return UBP_OK;
   UBP_API_END
 }

@@ -482,6 +482,13 @@ __global__ UBP_LB void k_xsph_drift(DevParams P, const float4* __restrict__ pos,
 }
 
 // ---- diagnostics ----
+// Plain streaming copy (HBM counter calibration).
+__global__ UBP_LB void k_calib_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n4) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (; i < n4; i += stride) dst[i] = src[i];
+}
+
 // Cell key of every slot's current position, scattered to original order.
 __global__ UBP_LB void k_debug_keys(DevParams P, const float4* __restrict__ pos,
                                     const uint32_t* __restrict__ orig, uint32_t* __restrict__ out) {
