@@ -107,6 +107,8 @@ def lib():
     L.orc_sdf_gradient.argtypes = [fp, C.c_float, u32p, fp, fp, fp]
     L.orc_neighbors.argtypes = [C.c_size_t, fp, fp, fp, C.c_float, fp, fp, C.c_float,
                                 C.c_size_t, u32p, C.c_size_t]
+    L.orc_check_div_h.argtypes = [C.c_float, C.c_float, C.c_float]
+    L.orc_check_div_h.restype = C.c_long
     _lib = L
     return L
 
@@ -273,6 +275,10 @@ def wall_omega(s, precision=64):
 
 def wall_gamma(s, precision=64):
     return lib().orc_wall_gamma(precision, float(s))
+
+
+def check_div_h(h, lo, hi) -> int:
+    return int(lib().orc_check_div_h(float(h), float(lo), float(hi)))
 
 
 def det_log(x):

@@ -954,6 +954,28 @@ int orc_restore_checkpoint(void* h) {
 }
 
 // ---- scalar entry points for closed-form tests ----
+// Number of floats r in [lo, hi] for which RN(r * RN(1/h)) corrected once by
+// FMA differs from the IEEE quotient r / h (the engine's division by h).
+long orc_check_div_h(float h, float lo, float hi) {
+  float inv = 1.0f / h;
+  uint32_t a, b;
+  std::memcpy(&a, &lo, 4);
+  std::memcpy(&b, &hi, 4);
+  long bad = 0;
+#pragma omp parallel for reduction(+ : bad) schedule(static)
+  for (long u = long(a); u <= long(b); ++u) {
+    uint32_t bits = uint32_t(u);
+    float r;
+    std::memcpy(&r, &bits, 4);
+    float q = r / h;
+    float q0 = r * inv;
+    float rem = std::fma(-q0, h, r);
+    float q1 = std::fma(rem, inv, q0);
+    if (std::memcmp(&q, &q1, 4) != 0) ++bad;
+  }
+  return bad;
+}
+
 double orc_wendland(int prec, double r, double h) {
   return prec == 64 ? wendland<double>(r, h) : double(wendland<float>(float(r), float(h)));
 }

@@ -221,3 +221,46 @@ def test_config2_dam_break_1m_first_steps():
     assert np.array_equal(np.sort(e.debug_bins()[1]), np.arange(1 << 20, dtype=np.uint32))
     assert s["x"].min() >= 0.0 and s["x"].max() <= 1.0 and s["z"].min() >= 0.0
     assert e.error_metrics().mass_conservation < 1e-12
+
+
+def test_boundary_particles_water_box_bitwise():
+    """Reference wall model (3-layer boundary particles, Adami pressures,
+    repulsion, domain clamp; domain.rs:162-348) on the engine vs the oracle."""
+    say("case: water box 6^3 with 3-layer wall particles, 40 device-dt steps")
+    sc = scenes.water_box_walls(n_side=6, dx=1e-3)
+    e = engine(sc)
+    o = oracle(sc)
+    e.step_n(40, use_graph=False)
+    o.step_auto(40)
+    assert_bitwise(e.particles(), o, "water box with wall particles after 40 steps")
+
+
+def test_boundary_particles_hydrostatic_column():
+    """kernel/tests/hydrostatic.rs:15-171 set-up on the engine, bitwise vs the
+    oracle after 300 fixed-dt steps, plus the reference's assertions."""
+    say("case: hydrostatic column 64 particles + 147 boundary, 300 steps")
+    h = 0.02
+    s = h / 1.3
+    pts = np.array([[(ix + 0.5) * s, (iy + 1.0) * s, (iz + 0.5) * s]
+                    for iy in range(4) for iz in range(4) for ix in range(4)])
+    bnd = np.array([[(ix + 0.5) * s, -layer * s, (iz + 0.5) * s]
+                    for layer in range(3) for iz in range(-1, 6) for ix in range(-1, 6)])
+    n, m = len(pts), 1000.0 * s ** 3
+    margin = 6 * h
+    sc = {"x": pts[:, 0], "y": pts[:, 1], "z": pts[:, 2], "mass": np.full(n, m),
+          "density": np.full(n, 1000.0), "h": np.float32(h), "gravity": (0.0, -9.81, 0.0),
+          "speed_of_sound": np.float32(20.0), "cfl_number": np.float32(0.2),
+          "domain_min": (-margin,) * 3,
+          "domain_max": (7 * s + margin, 14 * s + margin, 7 * s + margin),
+          "boundary": {"x": bnd[:, 0], "y": bnd[:, 1], "z": bnd[:, 2],
+                       "mass": np.full(len(bnd), m), "nx": np.zeros(len(bnd)),
+                       "ny": np.ones(len(bnd)), "nz": np.zeros(len(bnd))}}
+    e = engine(sc)
+    o = oracle(sc)
+    for _ in range(300):
+        e.step_async(1e-4)
+        o.step(np.float32(1e-4))
+    e.sync()
+    assert_bitwise(e.particles(), o, "hydrostatic column after 300 steps")
+    y = e.particles()["y"]
+    assert y.min() > -2 * s and y.max() - y.min() > 0.5 * s

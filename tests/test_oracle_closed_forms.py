@@ -183,3 +183,11 @@ def test_wall_polynomials_fp32_equal_fp64_at_dyadic_points():
         a = np.float32(po.wall_omega(s, 32))
         b = np.float32(po.wall_omega(s, 64))
         assert abs(float(a) - float(b)) <= 2 * np.spacing(np.float32(max(abs(b), 1e-30)))
+
+
+@pytest.mark.parametrize("h", [np.float32(1.3 * 0.25 / 81), np.float32(0.65e-3),
+                               np.float32(1.3 * 0.25 / 16), np.float32(0.05)])
+def test_engine_division_by_h_is_ieee_exhaustive(h):
+    # the engine computes r / h as fma(fma(-q0, h, r), 1/h, q0), q0 = r * (1/h);
+    # every float r in [1e-30, 2h] must give the IEEE quotient (Markstein)
+    assert po.check_div_h(h, 1e-30, 2.0001 * float(h)) == 0

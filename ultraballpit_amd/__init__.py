@@ -275,6 +275,9 @@ class HipKernel:
     def step(self, dt: float):
         _check(lib().ubp_step(self._h, float(dt)), "step")
 
+    def step_async(self, dt: float):
+        _check(lib().ubp_step_async(self._h, float(dt)), "step_async")
+
     def step_no_sync(self, dt: float):
         _check(lib().ubp_step_async(self._h, float(dt)), "step_no_sync")
 

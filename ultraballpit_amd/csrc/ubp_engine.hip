@@ -79,7 +79,8 @@ struct Engine {
 
   float4 *pos = nullptr, *pos_t = nullptr, *vel = nullptr, *vel_t = nullptr, *acc = nullptr;
   float4 *wall = nullptr, *wall_t = nullptr;
-  float *rho_g = nullptr, *rho_n = nullptr, *prs = nullptr, *prho2 = nullptr;
+  float2* rho_g = nullptr;
+  float *rho_n = nullptr, *prs = nullptr, *prho2 = nullptr;
   uint8_t* ft[2] = {nullptr, nullptr};
   uint32_t* orig[2] = {nullptr, nullptr};
   int pp = 0;
@@ -92,13 +93,18 @@ struct Engine {
   float* pmax = nullptr;
   double* psum = nullptr;
   uint32_t* dbg = nullptr;
+  // boundary particles
+  float4* bpos = nullptr;
+  float* bprs = nullptr;
+  uint32_t *bl_start = nullptr, *bl = nullptr, *bc_start = nullptr, *bc = nullptr;
 
   // checkpoint
   bool has_ckpt = false;
   bool ck_needs_init = true;
   int ck_pp = 0;
   float4 *ck_pos = nullptr, *ck_vel = nullptr, *ck_acc = nullptr, *ck_wall = nullptr;
-  float *ck_rho = nullptr, *ck_prs = nullptr, *ck_prho2 = nullptr, *ck_rho_g = nullptr;
+  float *ck_rho = nullptr, *ck_prs = nullptr, *ck_prho2 = nullptr;
+  float2* ck_rho_g = nullptr;
   uint8_t* ck_ft = nullptr;
   uint32_t *ck_orig = nullptr, *ck_start = nullptr;
 
@@ -131,6 +137,7 @@ struct Engine {
     dfree(key); dfree(arr); dfree(cnt); dfree(start); dfree(bsum); dfree(stage_src);
     dfree(stage_orig); dfree(nbr); dfree(nn); dfree(spread); dfree(sdf); dfree(dtp); dfree(red);
     dfree(pmax); dfree(psum); dfree(dbg);
+    dfree(bpos); dfree(bprs); dfree(bl_start); dfree(bl); dfree(bc_start); dfree(bc);
     dfree(ck_pos); dfree(ck_vel); dfree(ck_acc); dfree(ck_wall); dfree(ck_rho); dfree(ck_prs);
     dfree(ck_prho2); dfree(ck_rho_g); dfree(ck_ft); dfree(ck_orig); dfree(ck_start);
     if (h_stage) { (void)hipHostFree(h_stage); h_stage = nullptr; }
@@ -173,8 +180,6 @@ struct Engine {
     HIPCHK(hipGetDeviceProperties(&prop, dev));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
       throw Error(UBP_ENODEV, std::string("device is ") + prop.gcnArchName + ", this build targets gfx950");
-    if (bin && bin->n > 0)
-      throw Error(UBP_EINVAL, "boundary particles are not supported by this build; use ubp_set_sdf walls");
     if (!(h > 0.0f)) throw Error(UBP_EINVAL, "h must be positive");
     n = pin ? pin->n : 0;
     if (n > (size_t(1) << 28)) throw Error(UBP_EOVERFLOW, "too many particles (max 2^28)");
@@ -183,6 +188,7 @@ struct Engine {
     P.n = uint32_t(n);
     P.K = 128;
     P.h = h;
+    P.inv_h = 1.0f / h;
     P.cs = cs;
     P.cfl = cfl;
     for (int a = 0; a < 3; ++a) { P.g[a] = g[a]; P.dmin[a] = dmin[a]; P.dmax[a] = dmax[a]; }
@@ -270,7 +276,93 @@ struct Engine {
     std::vector<uint32_t> iota(n);
     for (size_t i = 0; i < n; ++i) iota[i] = uint32_t(i);
     if (n) HIPCHK(hipMemcpy(orig[pp], iota.data(), n * 4, hipMemcpyHostToDevice));
+    P.nb = 0;
+    if (bin && bin->n > 0) setup_boundary(bin);
     HIPCHK(hipDeviceSynchronize());
+  }
+
+  // Static candidate lists for the boundary particles (boundary.rs:20-37).
+  // Fluid cell c lists every boundary particle whose distance to c's box
+  // (widened by 1e-4 cell, open towards the outside on edge cells, where
+  // clamped particles land) is below 2h: a superset of the boundary particles
+  // within 2h of any fluid particle binned into c, in ascending index.  The
+  // transpose (per boundary particle, the cells) drives the Adami pass.
+  void setup_boundary(const ubp_boundary_in* bin) {
+    const size_t nb = bin->n;
+    if (nb > (size_t(1) << 26)) throw Error(UBP_EOVERFLOW, "too many boundary particles");
+    if (!bin->x || !bin->y || !bin->z || !bin->mass) throw Error(UBP_EINVAL, "boundary x, y, z, mass required");
+    std::vector<float4> hb(nb);
+    for (size_t b = 0; b < nb; ++b) hb[b] = make_float4(bin->x[b], bin->y[b], bin->z[b], bin->mass[b]);
+    const double cs = double(P.cell), marg = 1e-4 * cs;
+    const double lim = cs * cs;
+    std::vector<std::pair<uint32_t, uint32_t>> pairs;  // (code, b)
+    std::vector<uint32_t> hc_start(nb + 1, 0);
+    std::vector<uint32_t> hc;
+    auto axis_cell_h = [&](float pcoord, int a) {
+      float c = std::floor((pcoord - P.dmin[a]) / P.cell);
+      if (!(c >= 0.0f)) c = 0.0f;
+      float hi = float(P.dims[a] - 1u);
+      if (c > hi) c = hi;
+      return int(c);
+    };
+    for (size_t b = 0; b < nb; ++b) {
+      const float bp[3] = {hb[b].x, hb[b].y, hb[b].z};
+      int cb[3];
+      for (int a = 0; a < 3; ++a) cb[a] = axis_cell_h(bp[a], a);
+      hc_start[b] = uint32_t(hc.size());
+      for (int cz = cb[2] - 2; cz <= cb[2] + 2; ++cz) {
+        if (cz < 0 || cz >= int(P.dims[2])) continue;
+        for (int cy = cb[1] - 2; cy <= cb[1] + 2; ++cy) {
+          if (cy < 0 || cy >= int(P.dims[1])) continue;
+          for (int cx = cb[0] - 2; cx <= cb[0] + 2; ++cx) {
+            if (cx < 0 || cx >= int(P.dims[0])) continue;
+            const int cc[3] = {cx, cy, cz};
+            double d2 = 0.0;
+            for (int a = 0; a < 3; ++a) {
+              double lo = double(P.dmin[a]) + cc[a] * cs - marg;
+              double hi = double(P.dmin[a]) + (cc[a] + 1) * cs + marg;
+              if (cc[a] == 0) lo = -1e300;
+              if (cc[a] == int(P.dims[a]) - 1) hi = 1e300;
+              double v = double(bp[a]);
+              double g = (v < lo) ? lo - v : (v > hi ? v - hi : 0.0);
+              d2 += g * g;
+            }
+            if (d2 < lim) {
+              uint32_t code = morton_host(uint32_t(cx), uint32_t(cy), uint32_t(cz), bits);
+              pairs.emplace_back(code, uint32_t(b));
+              hc.push_back(code);
+            }
+          }
+        }
+      }
+    }
+    hc_start[nb] = uint32_t(hc.size());
+    // CSR by fluid cell code, boundary ids ascending (stable counting sort)
+    std::vector<uint32_t> hl_start(size_t(P.ncodes) + 1, 0);
+    for (auto& pr : pairs) hl_start[pr.first + 1] += 1;
+    for (size_t c = 0; c < P.ncodes; ++c) hl_start[c + 1] += hl_start[c];
+    std::vector<uint32_t> hl(pairs.size());
+    std::vector<uint32_t> head(hl_start.begin(), hl_start.end() - 1);
+    for (auto& pr : pairs) hl[head[pr.first]++] = pr.second;
+    dalloc(&bpos, nb);
+    dalloc(&bprs, nb);
+    dalloc(&bl_start, hl_start.size());
+    dalloc(&bl, hl.size());
+    dalloc(&bc_start, hc_start.size());
+    dalloc(&bc, hc.size());
+    HIPCHK(hipMemcpy(bpos, hb.data(), nb * 16, hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(bprs, 0, nb * 4));
+    HIPCHK(hipMemcpy(bl_start, hl_start.data(), hl_start.size() * 4, hipMemcpyHostToDevice));
+    if (!hl.empty()) HIPCHK(hipMemcpy(bl, hl.data(), hl.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(bc_start, hc_start.data(), hc_start.size() * 4, hipMemcpyHostToDevice));
+    if (!hc.empty()) HIPCHK(hipMemcpy(bc, hc.data(), hc.size() * 4, hipMemcpyHostToDevice));
+    P.nb = uint32_t(nb);
+    P.bpos = bpos;
+    P.bprs = bprs;
+    P.bl_start = bl_start;
+    P.bl = bl;
+    P.bc_start = bc_start;
+    P.bc = bc;
   }
 
   void alloc_nbr() {
@@ -331,26 +423,42 @@ struct Engine {
     LAUNCHCHK("kernel launch");
   }
 
-  // counting sort of pos_t order into slot order (keys/arrivals already made)
-  void bin() {
+  // Records E[k] on the stream when an event table is given (profiling).
+  void rec(hipEvent_t* E, int k) {
+    if (E) HIPCHK(hipEventRecord(E[k], st));
+  }
+
+  void scan() {
     hipLaunchKernelGGL(k_scan_local, dim3(nblk_scan), dim3(256), 0, st, cnt, start, bsum, P.ncodes);
     LAUNCHCHK("k_scan_local");
     hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(256), 0, st, bsum, nblk_scan);
     LAUNCHCHK("k_scan_bsum");
     hipLaunchKernelGGL(k_scan_add, dim3(nblk_scan), dim3(256), 0, st, start, bsum, P.ncodes);
     LAUNCHCHK("k_scan_add");
+  }
+
+  // counting sort of pos_t order into slot order (keys/arrivals already made)
+  void bin(hipEvent_t* E = nullptr) {
+    scan();
+    rec(E, 3);
     const uint32_t g = grid_of(n);
     launch(k_scatter, g, P, key, arr, start, orig[pp], stage_src, stage_orig);
+    rec(E, 4);
     launch(k_rank_gather, g, P, stage_src, stage_orig, key, start, pos_t, vel_t, rho_n, wall_t, ft[pp],
            pos, vel, rho_g, wall, ft[pp ^ 1], orig[pp ^ 1]);
     pp ^= 1;
   }
 
-  void density_forces() {
+  void density_forces(hipEvent_t* E = nullptr) {
     const uint32_t g = grid_of(n);
+    rec(E, 5);
     launch(k_neigh, g, P, pos, start, nbr, nn);
+    rec(E, 6);
     launch(k_density, g, P, pos, rho_g, wall, ft[pp], start, nbr, nn, dtp, rho_n, prs, prho2);
+    if (P.nb) launch(k_adami, grid_of(P.nb), P, pos, rho_n, prs, orig[pp], start, bprs);
+    rec(E, 7);
     launch(k_forces, g, P, pos, vel, rho_n, prs, prho2, wall, ft[pp], start, nbr, nn, acc);
+    rec(E, 8);
   }
 
   // CpuKernel::compute_forces on the initial state (lib.rs:295-298)
@@ -361,13 +469,18 @@ struct Engine {
     needs_init = false;
   }
 
-  void step_body(int do_cfl) {
+  // lib.rs:300-334 with the dt on the device; E (10 events) brackets kernels
+  void step_body(int do_cfl, hipEvent_t* E = nullptr) {
     const uint32_t g = grid_of(n);
+    rec(E, 0);
     launch(k_kick, g, P, vel, acc, dtp);
+    rec(E, 1);
     launch(k_xsph_drift, g, P, pos, vel, rho_n, start, nbr, nn, dtp, pos_t, vel_t, wall_t, key, arr, cnt);
-    bin();
-    density_forces();
+    rec(E, 2);
+    bin(E);
+    density_forces(E);
     launch(k_kick2_cfl, g, P, vel, acc, dtp, red, 1, do_cfl);
+    rec(E, 9);
   }
 
   void set_dt(float dt) {
@@ -447,34 +560,7 @@ struct Engine {
     std::vector<hipEvent_t> ev(size_t(nsteps) * per, nullptr);
     try {
       for (auto& x : ev) HIPCHK(hipEventCreate(&x));
-      const uint32_t g = grid_of(n);
-      for (int s = 0; s < nsteps; ++s) {
-        hipEvent_t* E = &ev[size_t(s) * per];
-        HIPCHK(hipEventRecord(E[0], st));
-        launch(k_kick, g, P, vel, acc, dtp);
-        HIPCHK(hipEventRecord(E[1], st));
-        launch(k_xsph_drift, g, P, pos, vel, rho_n, start, nbr, nn, dtp, pos_t, vel_t, wall_t, key, arr, cnt);
-        HIPCHK(hipEventRecord(E[2], st));
-        hipLaunchKernelGGL(k_scan_local, dim3(nblk_scan), dim3(256), 0, st, cnt, start, bsum, P.ncodes);
-        hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(256), 0, st, bsum, nblk_scan);
-        hipLaunchKernelGGL(k_scan_add, dim3(nblk_scan), dim3(256), 0, st, start, bsum, P.ncodes);
-        LAUNCHCHK("scan");
-        HIPCHK(hipEventRecord(E[3], st));
-        launch(k_scatter, g, P, key, arr, start, orig[pp], stage_src, stage_orig);
-        HIPCHK(hipEventRecord(E[4], st));
-        launch(k_rank_gather, g, P, stage_src, stage_orig, key, start, pos_t, vel_t, rho_n, wall_t, ft[pp],
-               pos, vel, rho_g, wall, ft[pp ^ 1], orig[pp ^ 1]);
-        pp ^= 1;
-        HIPCHK(hipEventRecord(E[5], st));
-        launch(k_neigh, g, P, pos, start, nbr, nn);
-        HIPCHK(hipEventRecord(E[6], st));
-        launch(k_density, g, P, pos, rho_g, wall, ft[pp], start, nbr, nn, dtp, rho_n, prs, prho2);
-        HIPCHK(hipEventRecord(E[7], st));
-        launch(k_forces, g, P, pos, vel, rho_n, prs, prho2, wall, ft[pp], start, nbr, nn, acc);
-        HIPCHK(hipEventRecord(E[8], st));
-        launch(k_kick2_cfl, g, P, vel, acc, dtp, red, 1, 1);
-        HIPCHK(hipEventRecord(E[9], st));
-      }
+      for (int s = 0; s < nsteps; ++s) step_body(1, &ev[size_t(s) * per]);
       sync();
       for (int s = 0; s < nsteps; ++s) {
         hipEvent_t* E = &ev[size_t(s) * per];
@@ -604,7 +690,7 @@ struct Engine {
     cp(ck_rho, rho_n, n * 4);
     cp(ck_prs, prs, n * 4);
     cp(ck_prho2, prho2, n * 4);
-    cp(ck_rho_g, rho_g, n * 4);
+    cp(ck_rho_g, rho_g, n * 8);
     cp(ck_ft, ft[pp], n);
     cp(ck_orig, orig[pp], n * 4);
     cp(ck_start, start, (size_t(P.ncodes) + 1) * 4);
@@ -625,7 +711,7 @@ struct Engine {
     cp(rho_n, ck_rho, n * 4);
     cp(prs, ck_prs, n * 4);
     cp(prho2, ck_prho2, n * 4);
-    cp(rho_g, ck_rho_g, n * 4);
+    cp(rho_g, ck_rho_g, n * 8);
     cp(ft[pp], ck_ft, n);
     cp(orig[pp], ck_orig, n * 4);
     cp(start, ck_start, (size_t(P.ncodes) + 1) * 4);
@@ -792,29 +878,13 @@ int ubp_step_profiled(ubp_engine* e, float dt, ubp_step_profile* out) {
   g.stepped = true;
   g.set_dt(dt);
   if (g.needs_init) { g.bootstrap(); }
-  hipEvent_t ev[7];
+  hipEvent_t ev[10];
   for (auto& x : ev) HIPCHK(hipEventCreate(&x));
-  const uint32_t gr = g.grid_of(g.n);
-  HIPCHK(hipEventRecord(ev[0], g.st));
-  g.launch(ubp::k_kick, gr, g.P, g.vel, g.acc, g.dtp);
-  g.launch(ubp::k_xsph_drift, gr, g.P, g.pos, g.vel, g.rho_n, g.start, g.nbr, g.nn, g.dtp, g.pos_t,
-           g.vel_t, g.wall_t, g.key, g.arr, g.cnt);
-  HIPCHK(hipEventRecord(ev[1], g.st));
-  g.bin();
-  HIPCHK(hipEventRecord(ev[2], g.st));
-  g.launch(ubp::k_neigh, gr, g.P, g.pos, g.start, g.nbr, g.nn);
-  g.launch(ubp::k_density, gr, g.P, g.pos, g.rho_g, g.wall, g.ft[g.pp], g.start, g.nbr, g.nn, g.dtp,
-           g.rho_n, g.prs, g.prho2);
-  HIPCHK(hipEventRecord(ev[3], g.st));
-  g.launch(ubp::k_forces, gr, g.P, g.pos, g.vel, g.rho_n, g.prs, g.prho2, g.wall, g.ft[g.pp], g.start,
-           g.nbr, g.nn, g.acc);
-  HIPCHK(hipEventRecord(ev[4], g.st));
-  g.launch(ubp::k_kick2_cfl, gr, g.P, g.vel, g.acc, g.dtp, g.red, 1, 0);
-  HIPCHK(hipEventRecord(ev[5], g.st));
+  g.step_body(0, ev);
   g.sync();
   g.dt_valid = false;
-  float ms[5];
-  for (int k = 0; k < 5; ++k) HIPCHK(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
+  float ms[9];
+  for (int k = 0; k < 9; ++k) HIPCHK(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
   auto t1 = std::chrono::steady_clock::now();
   // readback leg of the reference profile: a full particles() copy
   {
@@ -828,11 +898,11 @@ int ubp_step_profiled(ubp_engine* e, float dt, ubp_step_profile* out) {
   }
   auto t2 = std::chrono::steady_clock::now();
   for (auto& x : ev) (void)hipEventDestroy(x);
-  out->integrate_us = uint64_t((ms[0] + ms[4]) * 1000.0f);
-  out->grid_build_us = uint64_t(ms[1] * 1000.0f);
-  out->density_us = uint64_t(ms[2] * 1000.0f);
-  out->forces_us = uint64_t(ms[3] * 1000.0f);
-  out->boundary_pressure_us = 0;
+  out->integrate_us = uint64_t((ms[0] + ms[1] + ms[8]) * 1000.0f);
+  out->grid_build_us = uint64_t((ms[2] + ms[3] + ms[4]) * 1000.0f);
+  out->density_us = uint64_t((ms[5] + ms[6]) * 1000.0f);
+  out->forces_us = uint64_t(ms[7] * 1000.0f);
+  out->boundary_pressure_us = 0;  // inside the density bracket (k_adami follows k_density)
   out->readback_us = uint64_t(std::chrono::duration<double, std::micro>(t2 - t1).count());
   out->total_us = uint64_t(std::chrono::duration<double, std::micro>(t2 - t0).count());
   return UBP_OK;

@@ -16,6 +16,7 @@ namespace ubp {
 
 #define UBP_LB __launch_bounds__(256)
 
+
 __global__ void k_set_dt(float* dtp, float dt) {
   if (threadIdx.x == 0 && blockIdx.x == 0) dtp[0] = dt;
 }
@@ -122,7 +123,7 @@ __global__ UBP_LB void k_rank_gather(DevParams P, const uint32_t* __restrict__ s
                                      const float* __restrict__ rho_src, const float4* __restrict__ wall_t,
                                      const uint8_t* __restrict__ ft_src,
                                      float4* __restrict__ pos, float4* __restrict__ vel,
-                                     float* __restrict__ rho_g, float4* __restrict__ wall,
+                                     float2* __restrict__ rho_g, float4* __restrict__ wall,
                                      uint8_t* __restrict__ ft_dst, uint32_t* __restrict__ orig_dst) {
   uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= P.n) return;
@@ -133,9 +134,15 @@ __global__ UBP_LB void k_rank_gather(DevParams P, const uint32_t* __restrict__ s
   uint32_t rank = 0;
   for (uint32_t k = b; k < e; ++k) rank += (stage_orig[k] < me) ? 1u : 0u;
   uint32_t t = b + rank;
-  pos[t] = pos_t[s];
+  const float4 p = pos_t[s];
+  const float rho = rho_src[s];
+  // delta-SPH volume m_j / rho_j of the old density (sph.rs:199), computed
+  // once per particle instead of once per pair (same bits)
+  float rd = rho;
+  if (P.diff_floor > 0.0f && !(rd >= P.diff_floor)) rd = P.diff_floor;
+  pos[t] = p;
   vel[t] = vel_t[s];
-  rho_g[t] = rho_src[s];
+  rho_g[t] = make_float2(rho, p.w / rd);
   wall[t] = wall_t[s];
   ft_dst[t] = ft_src[s];
   orig_dst[t] = me;
@@ -217,7 +224,7 @@ __global__ UBP_LB void k_neigh(DevParams P, const float4* __restrict__ pos,
 // sph.rs:112-160 (density), sph.rs:169-216 (diffusion), lib.rs:232-237,
 // eos.rs:39-47.
 __global__ UBP_LB void k_density(DevParams P, const float4* __restrict__ pos,
-                                 const float* __restrict__ rho_g, const float4* __restrict__ wall,
+                                 const float2* __restrict__ rho_g, const float4* __restrict__ wall,
                                  const uint8_t* __restrict__ ft, const uint32_t* __restrict__ start,
                                  const uint32_t* __restrict__ nbr, const uint32_t* __restrict__ nn,
                                  const float* __restrict__ dtp, float* __restrict__ rho_n,
@@ -227,7 +234,7 @@ __global__ UBP_LB void k_density(DevParams P, const float4* __restrict__ pos,
   if (i >= P.n) return;
   const float dt = dtp[0];
   float4 pi = pos[i];
-  const float rho_i = rho_g[i];
+  const float rho_i = rho_g[i].x;
   Kahan rho, dif;
   rho.init(pi.w * P.norm_h3);  // m_i W(0, h)
   dif.init(0.0f);
@@ -237,18 +244,31 @@ __global__ UBP_LB void k_density(DevParams P, const float4* __restrict__ pos,
     float dz = pi.z - q.z;
     float dist_sq = dx * dx + dy * dy + dz * dz;
     float r = sqrtf(dist_sq);
-    float qq = r / P.h;
+    float qq = div_h(r, P);
     float gx, gy, gz;
     wgrad_q(dx, dy, dz, r, qq, P, gx, gy, gz);
-    float rho_j = rho_g[j];
-    float rho_jd = rho_j;
-    if (P.diff_floor > 0.0f && !(rho_jd >= P.diff_floor)) rho_jd = P.diff_floor;
-    float v_j = q.w / rho_jd;
+    const float2 rj = rho_g[j];
+    const float rho_j = rj.x;
+    const float v_j = rj.y;
     float r_dot_grad = dx * gx + dy * gy + dz * gz;
     float density_diff = rho_i - rho_j;
     dif.add(v_j * density_diff * 2.0f * r_dot_grad / (dist_sq + P.eta_sq));
     rho.add(q.w * wendland_q(qq, P));
   });
+  if (P.nb) {  // boundary particles in ascending index (sph.rs:144-156)
+    const float sr = 2.0f * P.h;
+    uint32_t cx, cy, cz;
+    pos_to_cell(P, pi.x, pi.y, pi.z, cx, cy, cz);
+    uint32_t code = morton(P, cx, cy, cz);
+    for (uint32_t k = P.bl_start[code], ke = P.bl_start[code + 1u]; k < ke; ++k) {
+      float4 q = P.bpos[P.bl[k]];
+      float dx = pi.x - q.x;
+      float dy = pi.y - q.y;
+      float dz = pi.z - q.z;
+      float r = sqrtf(dx * dx + dy * dy + dz * dz);
+      if (r < sr) rho.add(q.w * wendland_q(div_h(r, P), P));
+    }
+  }
   const int f = ft[i];
   float4 w = wall[i];
   if (w.w != __builtin_inff()) {
@@ -287,7 +307,7 @@ __global__ UBP_LB void k_forces(DevParams P, const float4* __restrict__ pos,
     float dy = pi.y - q.y;
     float dz = pi.z - q.z;
     float r = sqrtf(dx * dx + dy * dy + dz * dz);
-    float qq = r / P.h;
+    float qq = div_h(r, P);
     float gx, gy, gz;
     wgrad_q(dx, dy, dz, r, qq, P, gx, gy, gz);
     float factor = nmi * q.w * (pi_over + prho2[j]);
@@ -306,10 +326,53 @@ __global__ UBP_LB void k_forces(DevParams P, const float4* __restrict__ pos,
       vx.add(vf * gx); vy.add(vf * gy); vz.add(vf * gz);
     }
   });
+  uint32_t bk0 = 0, bk1 = 0;
+  if (P.nb) {  // boundary pressure (sph.rs:376-408), ascending boundary index
+    const float sr = 2.0f * P.h;
+    uint32_t cx, cy, cz;
+    pos_to_cell(P, pi.x, pi.y, pi.z, cx, cy, cz);
+    uint32_t code = morton(P, cx, cy, cz);
+    bk0 = P.bl_start[code];
+    bk1 = P.bl_start[code + 1u];
+    const float pr = prs[i];
+    const float pic = (pr > 0.0f) ? pr : 0.0f;
+    const float pic_over = pic / (rho_i * rho_i);
+    const float brho = P.rho0[ft[i]];
+    for (uint32_t k = bk0; k < bk1; ++k) {
+      uint32_t b = P.bl[k];
+      float4 q = P.bpos[b];
+      float dx = pi.x - q.x;
+      float dy = pi.y - q.y;
+      float dz = pi.z - q.z;
+      float r = sqrtf(dx * dx + dy * dy + dz * dz);
+      if (r < sr) {
+        float gx, gy, gz;
+        wgrad_q(dx, dy, dz, r, div_h(r, P), P, gx, gy, gz);
+        float pb_over = P.bprs[b] / (brho * brho);
+        float factor = nmi * q.w * (pic_over + pb_over);
+        fx.add(factor * gx); fy.add(factor * gy); fz.add(factor * gz);
+      }
+    }
+  }
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   ax += fx.s / pi.w; ay += fy.s / pi.w; az += fz.s / pi.w;
   ax += vx.s; ay += vy.s; az += vz.s;
   ax += P.g[0]; ay += P.g[1]; az += P.g[2];
+  for (uint32_t k = bk0; k < bk1; ++k) {  // repulsion (boundary.rs:156-180)
+    float4 q = P.bpos[P.bl[k]];
+    float dx = pi.x - q.x;
+    float dy = pi.y - q.y;
+    float dz = pi.z - q.z;
+    float r = sqrtf(dx * dx + dy * dy + dz * dz);
+    if (r < P.r0 && r > 1.0e-12f) {
+      float sr0 = 1.0f - r / P.r0;
+      float fm = P.d_rep * sr0 * sr0 / P.r0;
+      float inv_r = 1.0f / r;
+      ax += fm * dx * inv_r;
+      ay += fm * dy * inv_r;
+      az += fm * dz * inv_r;
+    }
+  }
   float4 w = wall[i];
   if (w.w != __builtin_inff()) {
     float s = w.w / P.h;
@@ -328,6 +391,91 @@ __global__ UBP_LB void k_forces(DevParams P, const float4* __restrict__ pos,
     }
   }
   acc[i] = make_float4(ax, ay, az, 0.0f);
+}
+
+// Adami boundary pressure (boundary.rs:87-131): kernel-weighted fluid pressure
+// with hydrostatic correction, summed over fluid particles in ascending
+// original index (the reference scans all fluid particles in index order).
+// Accepted terms are collected, ordered by original index, then summed.
+#define UBP_ADAMI_CAP 192
+__global__ UBP_LB void k_adami(DevParams P, const float4* __restrict__ pos,
+                               const float* __restrict__ rho_n, const float* __restrict__ prs,
+                               const uint32_t* __restrict__ orig, const uint32_t* __restrict__ start,
+                               float* __restrict__ bprs_out) {
+  uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= P.nb) return;
+  const float sr = 2.0f * P.h;
+  const float4 q = P.bpos[b];
+  uint32_t id[UBP_ADAMI_CAP];
+  float tw[UBP_ADAMI_CAP], twp[UBP_ADAMI_CAP];
+  uint32_t cnt = 0;
+  bool overflow = false;
+  for (uint32_t k = P.bc_start[b], ke = P.bc_start[b + 1u]; k < ke; ++k) {
+    uint32_t code = P.bc[k];
+    for (uint32_t s = start[code], se = start[code + 1u]; s < se; ++s) {
+      float4 p = pos[s];
+      float dx = q.x - p.x;
+      float dy = q.y - p.y;
+      float dz = q.z - p.z;
+      float r = sqrtf(dx * dx + dy * dy + dz * dz);
+      if (r < sr) {
+        float w = wendland_q(div_h(r, P), P);
+        float gdr = P.g[0] * dx + P.g[1] * dy + P.g[2] * dz;
+        float pe = prs[s] + rho_n[s] * gdr;
+        if (cnt < UBP_ADAMI_CAP) {
+          // insertion by original index
+          uint32_t o = orig[s];
+          uint32_t m = cnt;
+          while (m > 0 && id[m - 1] > o) { id[m] = id[m - 1]; tw[m] = tw[m - 1]; twp[m] = twp[m - 1]; --m; }
+          id[m] = o; tw[m] = w; twp[m] = w * pe;
+          ++cnt;
+        } else {
+          overflow = true;
+        }
+      }
+    }
+  }
+  float wp = 0.0f, ws = 0.0f;
+  if (!overflow) {
+    for (uint32_t m = 0; m < cnt; ++m) { wp += twp[m]; ws += tw[m]; }
+  } else {
+    // selection by original index without storage: repeatedly take the
+    // smallest id above the last one taken
+    uint32_t last = 0;
+    bool first = true;
+    for (;;) {
+      uint32_t best = 0xffffffffu;
+      float bw = 0.0f, bwp = 0.0f;
+      for (uint32_t k = P.bc_start[b], ke = P.bc_start[b + 1u]; k < ke; ++k) {
+        uint32_t code = P.bc[k];
+        for (uint32_t s = start[code], se = start[code + 1u]; s < se; ++s) {
+          uint32_t o = orig[s];
+          if ((!first && o <= last) || o >= best) continue;
+          float4 p = pos[s];
+          float dx = q.x - p.x;
+          float dy = q.y - p.y;
+          float dz = q.z - p.z;
+          float r = sqrtf(dx * dx + dy * dy + dz * dz);
+          if (r < sr) {
+            float w = wendland_q(div_h(r, P), P);
+            float gdr = P.g[0] * dx + P.g[1] * dy + P.g[2] * dz;
+            float pe = prs[s] + rho_n[s] * gdr;
+            best = o; bw = w; bwp = w * pe;
+          }
+        }
+      }
+      if (best == 0xffffffffu) break;
+      wp += bwp; ws += bw;
+      last = best;
+      first = false;
+    }
+  }
+  float pb = 0.0f;
+  if (ws > 1.0e-12f) {
+    float v = wp / ws;
+    pb = (v > 0.0f) ? v : 0.0f;
+  }
+  bprs_out[b] = pb;
 }
 
 // v += a dt/2 (lib.rs:301-305)
@@ -438,7 +586,7 @@ __global__ UBP_LB void k_xsph_drift(DevParams P, const float4* __restrict__ pos,
     float dy = pi.y - q.y;
     float dz = pi.z - q.z;
     float r = sqrtf(dx * dx + dy * dy + dz * dz);
-    float w = wendland_q(r / P.h, P);
+    float w = wendland_q(div_h(r, P), P);
     float rho_avg = 0.5f * (rho_i + rho_n[j]);
     float den = (rho_avg >= 1.0f) ? rho_avg : 1.0f;
     float factor = q.w / den * w;
