@@ -30,19 +30,23 @@ from ultraballpit_amd import scenes  # noqa: E402
 
 METRIC = "particle-steps/s @1M dam break, %HBM roofline; surrogate/plain speedup @≤1% err"
 HBM_PEAK_GBS = 8000.0
-# Algorithmic (compulsory) bytes per particle per launch, FP32 SoA fields each
-# read or written once per pass that needs them; neighbour re-reads are not
-# counted.  Sum over the step = 280 B per particle-step.
+# Algorithmic (compulsory) bytes per particle per launch: FP32 fields each
+# read or written once by the kernel that needs them; neighbour re-reads and
+# the neighbour masks (a design choice, not algorithmic) are not counted.
 ALG_BYTES = {
-    "k_xsph_drift": 44 + 72,   # XSPH (R pos, vel, rho, m; W dv) + fused KDK/SDF/CFL drift
+    "k_kick": 36,              # R vel 12, acc 12; W vel 12
+    "k_xsph+k_drift": 108,     # XSPH: R pos, vel, rho, m, W dv (44); drift: R pos, vel, dv,
+                               # W pos, vel (60); W cell key (4)
     "k_scan": 0,
-    "k_scatter": 8,            # R key, W perm
-    "k_rank_gather": 64,       # SoA gather R+W pos, vel, rho, m
-    "k_neigh": 16,             # R pos; W key (cell key pass)
+    "k_scatter": 8,            # R key, W permutation
+    "k_rank_gather": 64,       # SoA gather, R + W pos, vel, rho, m
+    "k_neigh": 12,             # R pos
     "k_density": 28,           # R pos 12, rho_old 4, m 4; W rho 4, P 4
     "k_forces": 48,            # R pos 12, vel 12, rho 4, P 4, m 4; W acc 12
-    "k_kick": 0, "k_kick2_cfl": 0,
+    "k_kick2_cfl": 36,         # R vel 12, acc 12; W vel 12
 }
+# Whole step, compulsory traffic with the kicks fused into one KDK pass
+# (key 16 + sort 8 + gather 64 + density 28 + forces 48 + XSPH 44 + KDK 72).
 STEP_ALG_BYTES = 280
 
 

@@ -139,7 +139,7 @@ int ubp_step_profiled(ubp_engine *e, float dt, ubp_step_profile *out);
  * benches/gpu_throughput.rs:136-161).  HIP events are recorded on the
  * handle's stream around every launch; kernel_ms[k] receives the summed
  * milliseconds of kernel k over the n steps, in the order:
- *   0 kick, 1 xsph_drift, 2 scan (3 launches), 3 scatter, 4 rank_gather,
+ *   0 kick, 1 xsph + drift, 2 scan (3 launches), 3 scatter, 4 rank_gather,
  *   5 neigh, 6 density, 7 forces, 8 kick2_cfl.
  * Synchronous. */
 #define UBP_NKERN 9

@@ -176,33 +176,37 @@ __device__ __forceinline__ bool sdf_sample(const DevParams& P, float px, float p
   out = c0 * (1.0f - fx) + c1 * fx;
   return true;
 }
-__device__ __forceinline__ bool sdf_gradient(const DevParams& P, float px, float py, float pz,
-                                             float& nx, float& ny, float& nz) {
-  float hh = P.scell * 0.5f;
-  float two_h = 2.0f * hh;
-  float a, b;
-  if (!sdf_sample(P, px + hh, py, pz, a) || !sdf_sample(P, px - hh, py, pz, b)) return false;
-  float dx = (a - b) / two_h;
-  if (!sdf_sample(P, px, py + hh, pz, a) || !sdf_sample(P, px, py - hh, pz, b)) return false;
-  float dy = (a - b) / two_h;
-  if (!sdf_sample(P, px, py, pz + hh, a) || !sdf_sample(P, px, py, pz - hh, b)) return false;
-  float dz = (a - b) / two_h;
-  float mag = sqrtf(dx * dx + dy * dy + dz * dz);
-  if (mag > 1e-8f) { nx = dx / mag; ny = dy / mag; nz = dz / mag; }
-  else { nx = 0.0f; ny = 0.0f; nz = 0.0f; }
-  return true;
-}
 // Wall record at a position: (n, d) with d = +inf when the wall cannot act
-// (outside the SDF grid, or d >= 2h so that no wall term is non-zero).
-__device__ __forceinline__ float4 wall_at(const DevParams& P, float px, float py, float pz) {
-  float4 w = make_float4(0.0f, 0.0f, 0.0f, __builtin_inff());
-  if (!P.sdf_on) return w;
+// (outside the SDF grid, or d >= 2h so that no wall term is non-zero).  The
+// normal is the normalised central difference at +-cell/2 (geometry.rs:
+// 153-175); any sample leaving the grid disables the wall.  Out of line: it is
+// evaluated once or twice per particle, not per pair.
+__device__ __noinline__ float4 wall_at(const DevParams& P, float px, float py, float pz) {
+  const float inf = __builtin_inff();
+  if (!P.sdf_on) return make_float4(0.0f, 0.0f, 0.0f, inf);
   float d;
-  if (!sdf_sample(P, px, py, pz, d)) return w;
+  if (!sdf_sample(P, px, py, pz, d)) return make_float4(0.0f, 0.0f, 0.0f, inf);
   float s = d / P.h;
-  if (!(s < 2.0f)) return w;
-  float nx, ny, nz;
-  if (!sdf_gradient(P, px, py, pz, nx, ny, nz)) return w;
+  if (!(s < 2.0f)) return make_float4(0.0f, 0.0f, 0.0f, inf);
+  const float hh = P.scell * 0.5f;
+  const float two_h = 2.0f * hh;
+  float g[3];
+#pragma unroll 1
+  for (int a = 0; a < 3; ++a) {
+    float v[2];
+#pragma unroll 1
+    for (int sgn = 0; sgn < 2; ++sgn) {
+      const float off = sgn == 0 ? hh : -hh;
+      const float qx = a == 0 ? px + off : px;
+      const float qy = a == 1 ? py + off : py;
+      const float qz = a == 2 ? pz + off : pz;
+      if (!sdf_sample(P, qx, qy, qz, v[sgn])) return make_float4(0.0f, 0.0f, 0.0f, inf);
+    }
+    g[a] = (v[0] - v[1]) / two_h;
+  }
+  float mag = sqrtf(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+  float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+  if (mag > 1e-8f) { nx = g[0] / mag; ny = g[1] / mag; nz = g[2] / mag; }
   return make_float4(nx, ny, nz, d);
 }
 

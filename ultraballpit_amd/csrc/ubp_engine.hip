@@ -78,7 +78,7 @@ struct Engine {
   uint32_t nblk_scan = 0;
 
   float4 *pos = nullptr, *pos_t = nullptr, *vel = nullptr, *vel_t = nullptr, *acc = nullptr;
-  float4 *wall = nullptr, *wall_t = nullptr;
+  float4 *wall = nullptr, *wall_t = nullptr, *dv = nullptr;
   float2* rho_g = nullptr;
   float *rho_n = nullptr, *prs = nullptr, *prho2 = nullptr;
   uint8_t* ft[2] = {nullptr, nullptr};
@@ -131,7 +131,7 @@ struct Engine {
     if (dev >= 0) (void)hipSetDevice(dev);
     if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
     if (st) { (void)hipStreamSynchronize(st); }
-    dfree(pos); dfree(pos_t); dfree(vel); dfree(vel_t); dfree(acc); dfree(wall); dfree(wall_t);
+    dfree(pos); dfree(pos_t); dfree(vel); dfree(vel_t); dfree(acc); dfree(wall); dfree(wall_t); dfree(dv);
     dfree(rho_g); dfree(rho_n); dfree(prs); dfree(prho2);
     for (int k = 0; k < 2; ++k) { dfree(ft[k]); dfree(orig[k]); }
     dfree(key); dfree(arr); dfree(cnt); dfree(start); dfree(bsum); dfree(stage_src);
@@ -247,7 +247,7 @@ struct Engine {
 
     // device state
     dalloc(&pos, n); dalloc(&pos_t, n); dalloc(&vel, n); dalloc(&vel_t, n); dalloc(&acc, n);
-    dalloc(&wall, n); dalloc(&wall_t, n);
+    dalloc(&wall, n); dalloc(&wall_t, n); dalloc(&dv, n);
     dalloc(&rho_g, n); dalloc(&rho_n, n); dalloc(&prs, n); dalloc(&prho2, n);
     for (int k = 0; k < 2; ++k) { dalloc(&ft[k], n); dalloc(&orig[k], n); }
     dalloc(&key, n); dalloc(&arr, n); dalloc(&stage_src, n); dalloc(&stage_orig, n); dalloc(&nn, n);
@@ -255,10 +255,11 @@ struct Engine {
     nblk_scan = (P.ncodes + 1023u) / 1024u;
     dalloc(&bsum, nblk_scan);
     alloc_nbr();
-    dalloc(&dtp, 4); dalloc(&red, 4);
+    dalloc(&dtp, 4);
+    dalloc(&red, 2 * ((n + 63) / 64 + 1));  // CFL partials, enough for 64-thread blocks
     dalloc(&pmax, (n + 255) / 256 + 1); dalloc(&psum, 2 * ((n + 255) / 256 + 1));
     HIPCHK(hipMemset(cnt, 0, (size_t(P.ncodes) + 1) * 4));
-    HIPCHK(hipMemset(red, 0, 16));
+    HIPCHK(hipMemset(red, 0, 8 * ((n + 63) / 64 + 1)));
     HIPCHK(hipMemset(dtp, 0, 16));
     uint32_t total = uint32_t(n);
     HIPCHK(hipMemcpy(start + P.ncodes, &total, 4, hipMemcpyHostToDevice));  // start[ncodes] = N
@@ -367,8 +368,7 @@ struct Engine {
 
   void alloc_nbr() {
     dfree(nbr);
-    size_t cap = size_t(P.K) * n;
-    dalloc(&nbr, cap);
+    dalloc(&nbr, size_t(27) * n);  // per-cell neighbour masks, column-major
   }
 
   void invalidate_graph() {
@@ -407,13 +407,7 @@ struct Engine {
       if (bs != 64 && bs != 128 && bs != 256) throw Error(UBP_EINVAL, "block size must be 64, 128 or 256");
       block = bs;
     }
-    if (K && K != P.K) {
-      if (K > 1024) throw Error(UBP_EINVAL, "neighbour capacity must be <= 1024");
-      HIPCHK(hipStreamSynchronize(st));
-      P.K = K;
-      alloc_nbr();
-      if (!needs_init) launch(k_neigh, grid_of(n), P, pos, start, nbr, nn);  // rebuild lists
-    }
+    (void)K;  // neighbour structure is fixed-size (27 masks per particle)
     invalidate_graph();
   }
 
@@ -475,12 +469,19 @@ struct Engine {
     rec(E, 0);
     launch(k_kick, g, P, vel, acc, dtp);
     rec(E, 1);
-    launch(k_xsph_drift, g, P, pos, vel, rho_n, start, nbr, nn, dtp, pos_t, vel_t, wall_t, key, arr, cnt);
+    launch(k_xsph, g, P, pos, vel, rho_n, start, nbr, nn, dv);
+    launch(k_drift, g, P, pos, vel, dv, dtp, pos_t, vel_t, wall_t, key, arr, cnt);
     rec(E, 2);
     bin(E);
     density_forces(E);
     launch(k_kick2_cfl, g, P, vel, acc, dtp, red, 1, do_cfl);
+    if (do_cfl) cfl_finalize();
     rec(E, 9);
+  }
+
+  void cfl_finalize() {
+    hipLaunchKernelGGL(k_cfl_finalize, dim3(1), dim3(256), 0, st, P, (const uint32_t*)red, grid_of(n), dtp);
+    LAUNCHCHK("k_cfl_finalize");
   }
 
   void set_dt(float dt) {
@@ -501,9 +502,12 @@ struct Engine {
     if (dt_valid) return;
     // compute_timestep on the current state (initial arrays before bootstrap)
     const float4* v = needs_init ? vel_t : vel;
-    hipLaunchKernelGGL(k_kick2_cfl, dim3(grid_of(n)), dim3(block), 0, st, P, (float4*)v, (const float4*)acc,
-                       dtp, red, 0, 1);
-    LAUNCHCHK("k_kick2_cfl(cfl only)");
+    if (n > 0) {
+      hipLaunchKernelGGL(k_kick2_cfl, dim3(grid_of(n)), dim3(block), 0, st, P, (float4*)v, (const float4*)acc,
+                         (const float*)dtp, red, 0, 1);
+      LAUNCHCHK("k_kick2_cfl(cfl only)");
+    }
+    cfl_finalize();  // with no particles: max signal = c_s, no force limit
     dt_valid = true;
   }
 

@@ -4,12 +4,12 @@
 //   pos   float4 (x, y, z, m)      vel  float4 (vx, vy, vz, 0)
 //   acc   float4 (ax, ay, az, 0)   wall float4 (nx, ny, nz, d) SDF record
 //   rho_g old density (gathered)   rho_n new density, prs, prho2 = P/rho^2
-//   ft u8, orig u32 (ping-pong)    nbr u32 [K][N] column-major neighbour lists
+//   ft u8, orig u32 (ping-pong)    nmask u32 [27][N] per-cell accepted-neighbour masks
 // One step (paths relative to the reference repository; CPU step
 // backend/crates/kernel/src/lib.rs:287-335):
-//   kick1 -> xsph_drift (+ SDF projection, domain clamp, cell key, histogram)
+//   kick1 -> xsph -> drift (+ SDF projection, domain clamp, cell key, histogram)
 //   -> scan -> scatter -> rank_gather -> neigh -> density -> forces
-//   -> kick2 (+ CFL reduction for the next dt, dt stays on the device).
+//   -> kick2 (+ CFL partials) -> cfl_finalize (next dt, stays on the device).
 #include "ubp_device.hpp"
 
 namespace ubp {
@@ -19,6 +19,25 @@ namespace ubp {
 
 __global__ void k_set_dt(float* dtp, float dt) {
   if (threadIdx.x == 0 && blockIdx.x == 0) dtp[0] = dt;
+}
+
+// Arrival rank of this lane's particle in cell `code`: one atomic per
+// distinct code in the wavefront (ballot/shuffle), ranks by lane order.
+__device__ __forceinline__ uint32_t wave_histogram_add(uint32_t* __restrict__ cnt, uint32_t code) {
+  const uint32_t lane = threadIdx.x & 63u;
+  uint64_t remaining = __ballot(1);
+  uint32_t arrival = 0;
+  while (remaining) {
+    const int leader = __ffsll((unsigned long long)remaining) - 1;
+    const uint32_t lc = __shfl(code, leader, 64);
+    const uint64_t same = __ballot(code == lc) & remaining;
+    uint32_t base = 0;
+    if (int(lane) == leader) base = atomicAdd(&cnt[lc], uint32_t(__popcll(same)));
+    base = __shfl(base, leader, 64);
+    if ((same >> lane) & 1ull) arrival = base + uint32_t(__popcll(same & ((1ull << lane) - 1ull)));
+    remaining &= ~same;
+  }
+  return arrival;
 }
 
 // Cell key + histogram for positions in the current (unsorted) order; also the
@@ -33,7 +52,7 @@ __global__ UBP_LB void k_key_count(DevParams P, const float4* __restrict__ pos_t
   pos_to_cell(P, p.x, p.y, p.z, cx, cy, cz);
   uint32_t code = morton(P, cx, cy, cz);
   key[i] = code;
-  arr[i] = atomicAdd(&cnt[code], 1u);
+  arr[i] = wave_histogram_add(cnt, code);
   wall_t[i] = wall_at(P, p.x, p.y, p.z);
 }
 
@@ -182,42 +201,127 @@ __device__ __forceinline__ void walk27(const DevParams& P, uint32_t i, float4 pi
   }
 }
 
-// Neighbours of slot i in reference order: from the list when it fits,
-// otherwise by re-walking the cells (identical order).
+// Neighbour structure: for every particle and each of its 27 cells (in the
+// reference visiting order dz, dy, dx; neighbor.rs:147-183) a 32-bit mask of
+// the accepted slots of that cell (bit b = slot start[cell] + b).  Stored
+// column-major, nmask[k * N + i], so every store and load is coalesced.
+// nn[i] holds the accepted count; bit 31 flags a cell with more than 32
+// particles, for which the passes re-walk the cells instead.
+#define UBP_NN_OVF 0x80000000u
+
+// Spread values of the 3 x 3 x 3 neighbourhood of (cx, cy, cz); out-of-grid
+// axes get 0xffffffff.
+struct Nbhd {
+  uint32_t sx[3], sy[3], sz[3];
+};
+__device__ __forceinline__ void nbhd(const DevParams& P, uint32_t cx, uint32_t cy, uint32_t cz, Nbhd& N) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    int x = int(cx) + d - 1, y = int(cy) + d - 1, z = int(cz) + d - 1;
+    N.sx[d] = (x >= 0 && x < int(P.dims[0])) ? P.spread[x] : 0xffffffffu;
+    N.sy[d] = (y >= 0 && y < int(P.dims[1])) ? P.spread[P.dims[0] + y] : 0xffffffffu;
+    N.sz[d] = (z >= 0 && z < int(P.dims[2])) ? P.spread[P.dims[0] + P.dims[1] + z] : 0xffffffffu;
+  }
+}
+
+__device__ __forceinline__ uint32_t sel3(const uint32_t v[3], int d) {
+  return d == 0 ? v[0] : (d == 1 ? v[1] : v[2]);
+}
+
+// Neighbours of slot i in reference order.  One rolled loop over the 27
+// cells with a single copy of the pair body: normally one 32-slot chunk per
+// cell whose mask comes from nmask; for an overflowed particle every cell is
+// re-tested in 32-slot chunks (same order, same acceptance test).
 template <class F>
 __device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, float4 pi,
                                               const float4* __restrict__ pos,
                                               const uint32_t* __restrict__ start,
-                                              const uint32_t* __restrict__ nbr,
+                                              const uint32_t* __restrict__ nmask,
                                               const uint32_t* __restrict__ nn, F&& f) {
-  uint32_t c = nn[i];
-  if (c <= P.K) {
-    const uint32_t* l = nbr + i;
-    for (uint32_t k = 0; k < c; ++k) {
-      uint32_t j = l[size_t(k) * P.n];
-      f(j, pos[j]);
+  const bool ovf = (nn[i] & UBP_NN_OVF) != 0u;
+  uint32_t cx, cy, cz;
+  pos_to_cell(P, pi.x, pi.y, pi.z, cx, cy, cz);
+  Nbhd N;
+  nbhd(P, cx, cy, cz, N);
+  const uint32_t* m = nmask + i;
+#pragma unroll 1
+  for (int k = 0; k < 27; ++k) {
+    const int dz = k / 9, dy = (k / 3) % 3, dx = k % 3;
+    const uint32_t sx = sel3(N.sx, dx), sy = sel3(N.sy, dy), sz = sel3(N.sz, dz);
+    uint32_t mask0 = 0u;
+    if (!ovf) {
+      mask0 = m[size_t(k) * P.n];
+      if (mask0 == 0u) continue;
+    } else if (sx == 0xffffffffu || sy == 0xffffffffu || sz == 0xffffffffu) {
+      continue;
     }
-  } else {
-    walk27(P, i, pi, pos, start, f);
+    const uint32_t code = sx | sy | sz;
+    const uint32_t b = start[code];
+    const uint32_t e = ovf ? start[code + 1u] : b + 1u;
+    for (uint32_t c0 = b; c0 < e; c0 += 32u) {
+      uint32_t mask = mask0;
+      if (ovf) {
+        const uint32_t ce = (e - c0 > 32u) ? c0 + 32u : e;
+        for (uint32_t s = c0; s < ce; ++s) {
+          if (s == i) continue;
+          float4 q = pos[s];
+          float ddx = pi.x - q.x;
+          float ddy = pi.y - q.y;
+          float ddz = pi.z - q.z;
+          float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+          if (d2 <= P.rsq) mask |= 1u << (s - c0);
+        }
+      }
+      while (mask) {
+        const uint32_t bit = uint32_t(__builtin_ctz(mask));
+        mask &= mask - 1u;
+        const uint32_t j = c0 + bit;
+        f(j, pos[j]);
+      }
+    }
   }
 }
 
-// Neighbour lists (cell walk once per step; reused by density, forces and the
-// next step's XSPH, which all see the same positions).
+// Cell walk once per step; the masks are reused by density, forces and the
+// next step's XSPH, which all see the same positions.
 __global__ UBP_LB void k_neigh(DevParams P, const float4* __restrict__ pos,
-                               const uint32_t* __restrict__ start, uint32_t* __restrict__ nbr,
+                               const uint32_t* __restrict__ start, uint32_t* __restrict__ nmask,
                                uint32_t* __restrict__ nn) {
   uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   uint32_t i = blk * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
-  float4 pi = pos[i];
-  uint32_t c = 0;
-  uint32_t* l = nbr + i;
-  walk27(P, i, pi, pos, start, [&](uint32_t j, float4) {
-    if (c < P.K) l[size_t(c) * P.n] = j;
-    ++c;
-  });
-  nn[i] = c;
+  const float4 pi = pos[i];
+  uint32_t cx, cy, cz;
+  pos_to_cell(P, pi.x, pi.y, pi.z, cx, cy, cz);
+  Nbhd N;
+  nbhd(P, cx, cy, cz, N);
+  uint32_t c = 0, ovf = 0;
+  uint32_t* m = nmask + i;
+#pragma unroll 1
+  for (int k = 0; k < 27; ++k) {
+    const int dz = k / 9, dy = (k / 3) % 3, dx = k % 3;
+    const uint32_t sx = sel3(N.sx, dx), sy = sel3(N.sy, dy), sz = sel3(N.sz, dz);
+    uint32_t mask = 0u;
+    if (sx != 0xffffffffu && sy != 0xffffffffu && sz != 0xffffffffu) {
+      const uint32_t code = sx | sy | sz;
+      const uint32_t b = start[code], e = start[code + 1u];
+      if (e - b > 32u) ovf = UBP_NN_OVF;
+      for (uint32_t s = b; s < e; ++s) {
+        if (s == i) continue;
+        float4 q = pos[s];
+        float ddx = pi.x - q.x;
+        float ddy = pi.y - q.y;
+        float ddz = pi.z - q.z;
+        float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        if (d2 <= P.rsq) {
+          ++c;
+          if (s - b < 32u) mask |= 1u << (s - b);
+        }
+      }
+    }
+    m[size_t(k) * P.n] = mask;
+  }
+  nn[i] = c | ovf;
 }
 
 // Density summation + delta-SPH (old density) + SDF wall volume + Tait EOS.
@@ -226,7 +330,7 @@ __global__ UBP_LB void k_neigh(DevParams P, const float4* __restrict__ pos,
 __global__ UBP_LB void k_density(DevParams P, const float4* __restrict__ pos,
                                  const float2* __restrict__ rho_g, const float4* __restrict__ wall,
                                  const uint8_t* __restrict__ ft, const uint32_t* __restrict__ start,
-                                 const uint32_t* __restrict__ nbr, const uint32_t* __restrict__ nn,
+                                 const uint32_t* __restrict__ nmask, const uint32_t* __restrict__ nn,
                                  const float* __restrict__ dtp, float* __restrict__ rho_n,
                                  float* __restrict__ prs, float* __restrict__ prho2) {
   uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
@@ -238,7 +342,7 @@ __global__ UBP_LB void k_density(DevParams P, const float4* __restrict__ pos,
   Kahan rho, dif;
   rho.init(pi.w * P.norm_h3);  // m_i W(0, h)
   dif.init(0.0f);
-  for_neighbors(P, i, pi, pos, start, nbr, nn, [&](uint32_t j, float4 q) {
+  for_neighbors(P, i, pi, pos, start, nmask, nn, [&](uint32_t j, float4 q) {
     float dx = pi.x - q.x;
     float dy = pi.y - q.y;
     float dz = pi.z - q.z;
@@ -289,7 +393,7 @@ __global__ UBP_LB void k_forces(DevParams P, const float4* __restrict__ pos,
                                 const float4* __restrict__ vel, const float* __restrict__ rho_n,
                                 const float* __restrict__ prs, const float* __restrict__ prho2,
                                 const float4* __restrict__ wall, const uint8_t* __restrict__ ft,
-                                const uint32_t* __restrict__ start, const uint32_t* __restrict__ nbr,
+                                const uint32_t* __restrict__ start, const uint32_t* __restrict__ nmask,
                                 const uint32_t* __restrict__ nn, float4* __restrict__ acc) {
   uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   uint32_t i = blk * blockDim.x + threadIdx.x;
@@ -302,7 +406,7 @@ __global__ UBP_LB void k_forces(DevParams P, const float4* __restrict__ pos,
   Kahan fx, fy, fz, vx, vy, vz;
   fx.init(0.0f); fy.init(0.0f); fz.init(0.0f);
   vx.init(0.0f); vy.init(0.0f); vz.init(0.0f);
-  for_neighbors(P, i, pi, pos, start, nbr, nn, [&](uint32_t j, float4 q) {
+  for_neighbors(P, i, pi, pos, start, nmask, nn, [&](uint32_t j, float4 q) {
     float dx = pi.x - q.x;
     float dy = pi.y - q.y;
     float dz = pi.z - q.z;
@@ -493,15 +597,13 @@ __global__ UBP_LB void k_kick(DevParams P, float4* __restrict__ vel, const float
 }
 
 // Second half-kick (lib.rs:330-334) fused with the CFL reduction of
-// compute_timestep (sph.rs:591-628) for the next step: max(|v| + c_s) and
-// max |a| as float bits (non-negative, so integer max == float max); the last
-// workgroup turns them into dt and re-arms the reduction slots.
-// red[0] = max signal bits, red[1] = max accel bits, red[2] = arrival counter.
+// compute_timestep (sph.rs:591-628) for the next step: per workgroup
+// max(|v| + c_s) and max |a| as float bits (non-negative, so integer max ==
+// float max) into part[2 * block + {0, 1}]; k_cfl_finalize turns them into dt.
 __global__ UBP_LB void k_kick2_cfl(DevParams P, float4* __restrict__ vel, const float4* __restrict__ acc,
-                                   float* __restrict__ dtp, uint32_t* __restrict__ red, int do_kick,
+                                   const float* __restrict__ dtp, uint32_t* __restrict__ part, int do_kick,
                                    int do_cfl) {
   __shared__ uint32_t smax[2][4];
-  __shared__ int last;
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const float half_dt = 0.5f * dtp[0];
   float sig = 0.0f, am = 0.0f;
@@ -537,17 +639,38 @@ __global__ UBP_LB void k_kick2_cfl(DevParams P, float4* __restrict__ vel, const 
       m0 = m0 > smax[0][k] ? m0 : smax[0][k];
       m1 = m1 > smax[1][k] ? m1 : smax[1][k];
     }
-    atomicMax(&red[0], m0);
-    atomicMax(&red[1], m1);
-    __threadfence();
-    uint32_t prev = atomicAdd(&red[2], 1u);
-    last = (prev == gridDim.x - 1u);
+    part[2 * blockIdx.x] = m0;
+    part[2 * blockIdx.x + 1] = m1;
   }
+}
+
+// dt = clamp(min(CFL h / max_signal, 0.25 sqrt(h / max|a|)), 1e-8, 0.01)
+// (sph.rs:617-627) from the per-workgroup partials; one workgroup.
+__global__ UBP_LB void k_cfl_finalize(DevParams P, const uint32_t* __restrict__ part, uint32_t nparts,
+                                      float* __restrict__ dtp) {
+  __shared__ uint32_t sm[2][4];
+  uint32_t m0 = 0, m1 = 0;
+  for (uint32_t k = threadIdx.x; k < nparts; k += blockDim.x) {
+    uint32_t a = part[2 * k], b = part[2 * k + 1];
+    m0 = m0 > a ? m0 : a;
+    m1 = m1 > b ? m1 : b;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t t0 = __shfl_xor(m0, o, 64);
+    uint32_t t1 = __shfl_xor(m1, o, 64);
+    m0 = m0 > t0 ? m0 : t0;
+    m1 = m1 > t1 ? m1 : t1;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) { sm[0][wv] = m0; sm[1][wv] = m1; }
   __syncthreads();
-  if (last && threadIdx.x == 0) {
-    __threadfence();
-    uint32_t m0 = atomicAdd(&red[0], 0u);
-    uint32_t m1 = atomicAdd(&red[1], 0u);
+  if (threadIdx.x == 0) {
+    m0 = 0u;
+    m1 = 0u;
+    for (int k = 0; k < int(blockDim.x >> 6); ++k) {
+      m0 = m0 > sm[0][k] ? m0 : sm[0][k];
+      m1 = m1 > sm[1][k] ? m1 : sm[1][k];
+    }
     float max_signal = __uint_as_float(m0);
     if (!(max_signal > P.cs)) max_signal = P.cs;  // starts at c_s
     float max_acc = __uint_as_float(m1);
@@ -557,31 +680,23 @@ __global__ UBP_LB void k_kick2_cfl(DevParams P, float4* __restrict__ vel, const 
     if (dt < 1.0e-8f) dt = 1.0e-8f;
     if (dt > 0.01f) dt = 0.01f;
     dtp[0] = dt;
-    red[0] = 0u;
-    red[1] = 0u;
-    red[2] = 0u;
   }
 }
 
-// XSPH (sph.rs:236-280) on the post-kick velocities, drift (lib.rs:313-317),
-// SDF wall projection, domain clamp (boundary.rs:239-291), then the wall
-// record, cell key and histogram of the new position (neighbor.rs:89-101).
-__global__ UBP_LB void k_xsph_drift(DevParams P, const float4* __restrict__ pos,
-                                    const float4* __restrict__ vel, const float* __restrict__ rho_n,
-                                    const uint32_t* __restrict__ start, const uint32_t* __restrict__ nbr,
-                                    const uint32_t* __restrict__ nn, const float* __restrict__ dtp,
-                                    float4* __restrict__ pos_t, float4* __restrict__ vel_t,
-                                    float4* __restrict__ wall_t, uint32_t* __restrict__ key,
-                                    uint32_t* __restrict__ arr, uint32_t* __restrict__ cnt) {
+// XSPH correction (sph.rs:236-280) on the post-kick velocities, over the
+// neighbour masks of the last density pass (same positions).
+__global__ UBP_LB void k_xsph(DevParams P, const float4* __restrict__ pos,
+                              const float4* __restrict__ vel, const float* __restrict__ rho_n,
+                              const uint32_t* __restrict__ start, const uint32_t* __restrict__ nmask,
+                              const uint32_t* __restrict__ nn, float4* __restrict__ dv) {
   uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   uint32_t i = blk * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
-  const float dt = dtp[0];
   float4 pi = pos[i];
   float4 vi = vel[i];
   const float rho_i = rho_n[i];
   float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-  for_neighbors(P, i, pi, pos, start, nbr, nn, [&](uint32_t j, float4 q) {
+  for_neighbors(P, i, pi, pos, start, nmask, nn, [&](uint32_t j, float4 q) {
     float dx = pi.x - q.x;
     float dy = pi.y - q.y;
     float dz = pi.z - q.z;
@@ -595,16 +710,33 @@ __global__ UBP_LB void k_xsph_drift(DevParams P, const float4* __restrict__ pos,
     cy += factor * (vj.y - vi.y);
     cz += factor * (vj.z - vi.z);
   });
-  float dvx = 0.5f * cx, dvy = 0.5f * cy, dvz = 0.5f * cz;
-  float x = pi.x + (vi.x + dvx) * dt;
-  float y = pi.y + (vi.y + dvy) * dt;
-  float z = pi.z + (vi.z + dvz) * dt;
+  dv[i] = make_float4(0.5f * cx, 0.5f * cy, 0.5f * cz, 0.0f);
+}
+
+// Drift (lib.rs:313-317), SDF wall projection, domain clamp (boundary.rs:
+// 239-291), then the wall record, cell key and histogram of the new position
+// (neighbor.rs:89-101).
+__global__ UBP_LB void k_drift(DevParams P, const float4* __restrict__ pos,
+                               const float4* __restrict__ vel, const float4* __restrict__ dv,
+                               const float* __restrict__ dtp, float4* __restrict__ pos_t,
+                               float4* __restrict__ vel_t, float4* __restrict__ wall_t,
+                               uint32_t* __restrict__ key, uint32_t* __restrict__ arr,
+                               uint32_t* __restrict__ cnt) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const float dt = dtp[0];
+  const float4 pi = pos[i];
+  const float4 vi = vel[i];
+  const float4 d = dv[i];
+  float x = pi.x + (vi.x + d.x) * dt;
+  float y = pi.y + (vi.y + d.y) * dt;
+  float z = pi.z + (vi.z + d.z) * dt;
   float vx = vi.x, vy = vi.y, vz = vi.z;
   if (P.sdf_on) {
     float4 w = wall_at(P, x, y, z);
     if (w.w != __builtin_inff() && w.w < 0.0f) {
-      float d = w.w;
-      x -= d * w.x; y -= d * w.y; z -= d * w.z;
+      float dd = w.w;
+      x -= dd * w.x; y -= dd * w.y; z -= dd * w.z;
       float vn = vx * w.x + vy * w.y + vz * w.z;
       if (vn < 0.0f) {
         float k = (1.0f + 0.2f) * vn;
@@ -626,7 +758,7 @@ __global__ UBP_LB void k_xsph_drift(DevParams P, const float4* __restrict__ pos,
   pos_to_cell(P, x, y, z, ccx, ccy, ccz);
   uint32_t code = morton(P, ccx, ccy, ccz);
   key[i] = code;
-  arr[i] = atomicAdd(&cnt[code], 1u);
+  arr[i] = wave_histogram_add(cnt, code);
 }
 
 // ---- diagnostics ----
