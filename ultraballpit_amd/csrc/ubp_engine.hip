@@ -368,7 +368,7 @@ struct Engine {
 
   void alloc_nbr() {
     dfree(nbr);
-    dalloc(&nbr, size_t(27) * n);  // per-cell neighbour masks, column-major
+    dalloc(&nbr, size_t(P.K) * n);  // neighbour lists, one row of K per particle
   }
 
   void invalidate_graph() {
@@ -407,7 +407,13 @@ struct Engine {
       if (bs != 64 && bs != 128 && bs != 256) throw Error(UBP_EINVAL, "block size must be 64, 128 or 256");
       block = bs;
     }
-    (void)K;  // neighbour structure is fixed-size (27 masks per particle)
+    if (K && K != P.K) {
+      if (K % 4 != 0 || K < 4 || K > 1024) throw Error(UBP_EINVAL, "neighbour capacity must be a multiple of 4 in [4, 1024]");
+      HIPCHK(hipStreamSynchronize(st));
+      P.K = K;
+      alloc_nbr();
+      if (!needs_init) launch(k_neigh, grid_of(n), P, pos, start, nbr, nn);  // rebuild lists
+    }
     invalidate_graph();
   }
 

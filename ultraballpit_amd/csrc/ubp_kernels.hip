@@ -4,7 +4,7 @@
 //   pos   float4 (x, y, z, m)      vel  float4 (vx, vy, vz, 0)
 //   acc   float4 (ax, ay, az, 0)   wall float4 (nx, ny, nz, d) SDF record
 //   rho_g old density (gathered)   rho_n new density, prs, prho2 = P/rho^2
-//   ft u8, orig u32 (ping-pong)    nmask u32 [27][N] per-cell accepted-neighbour masks
+//   ft u8, orig u32 (ping-pong)    nbr u32 [N][K] neighbour lists (row per particle)
 // One step (paths relative to the reference repository; CPU step
 // backend/crates/kernel/src/lib.rs:287-335):
 //   kick1 -> xsph -> drift (+ SDF projection, domain clamp, cell key, histogram)
@@ -167,47 +167,15 @@ __global__ UBP_LB void k_rank_gather(DevParams P, const uint32_t* __restrict__ s
   orig_dst[t] = me;
 }
 
-// Walk the 27 cells around particle i in the reference order (dz, dy, dx;
-// neighbor.rs:147-183), slots ascending (= original id ascending), skip i,
-// accept dist^2 <= (2h)^2.  f(j, pos_j) is called for accepted neighbours.
-template <class F>
-__device__ __forceinline__ void walk27(const DevParams& P, uint32_t i, float4 pi,
-                                       const float4* __restrict__ pos,
-                                       const uint32_t* __restrict__ start, F&& f) {
-  uint32_t cx, cy, cz;
-  pos_to_cell(P, pi.x, pi.y, pi.z, cx, cy, cz);
-  for (int dz = -1; dz <= 1; ++dz) {
-    int nz = int(cz) + dz;
-    if (nz < 0 || nz >= int(P.dims[2])) continue;
-    for (int dy = -1; dy <= 1; ++dy) {
-      int ny = int(cy) + dy;
-      if (ny < 0 || ny >= int(P.dims[1])) continue;
-      for (int dx = -1; dx <= 1; ++dx) {
-        int nx = int(cx) + dx;
-        if (nx < 0 || nx >= int(P.dims[0])) continue;
-        uint32_t code = morton(P, uint32_t(nx), uint32_t(ny), uint32_t(nz));
-        uint32_t b = start[code], e = start[code + 1u];
-        for (uint32_t s = b; s < e; ++s) {
-          if (s == i) continue;
-          float4 q = pos[s];
-          float ddx = pi.x - q.x;
-          float ddy = pi.y - q.y;
-          float ddz = pi.z - q.z;
-          float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-          if (d2 <= P.rsq) f(s, q);
-        }
-      }
-    }
-  }
-}
+// Neighbour lists: row i holds the accepted neighbour slots of particle i in
+// the reference visiting order (cells dz, dy, dx; slots ascending), up to K
+// entries: nbr[i * K + t].  A lane fills its own row front to back, so the
+// partial-line stores of a wavefront merge in L2 into whole lines.  nn[i] is
+// the accepted count; a count above K makes the passes re-walk the cells.
 
-// Neighbour structure: for every particle and each of its 27 cells (in the
-// reference visiting order dz, dy, dx; neighbor.rs:147-183) a 32-bit mask of
-// the accepted slots of that cell (bit b = slot start[cell] + b).  Stored
-// column-major, nmask[k * N + i], so every store and load is coalesced.
-// nn[i] holds the accepted count; bit 31 flags a cell with more than 32
-// particles, for which the passes re-walk the cells instead.
-#define UBP_NN_OVF 0x80000000u
+__device__ __forceinline__ uint32_t sel3(const uint32_t v[3], int d) {
+  return d == 0 ? v[0] : (d == 1 ? v[1] : v[2]);
+}
 
 // Spread values of the 3 x 3 x 3 neighbourhood of (cx, cy, cz); out-of-grid
 // axes get 0xffffffff.
@@ -224,104 +192,72 @@ __device__ __forceinline__ void nbhd(const DevParams& P, uint32_t cx, uint32_t c
   }
 }
 
-__device__ __forceinline__ uint32_t sel3(const uint32_t v[3], int d) {
-  return d == 0 ? v[0] : (d == 1 ? v[1] : v[2]);
-}
-
-// Neighbours of slot i in reference order.  One rolled loop over the 27
-// cells with a single copy of the pair body: normally one 32-slot chunk per
-// cell whose mask comes from nmask; for an overflowed particle every cell is
-// re-tested in 32-slot chunks (same order, same acceptance test).
+// Walk the 27 cells around particle i in the reference order (dz, dy, dx;
+// neighbor.rs:147-183), slots ascending (= original id ascending), skip i,
+// accept dist^2 <= (2h)^2; f(j, pos_j) for accepted neighbours.  Rolled: one
+// copy of the body.
 template <class F>
-__device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, float4 pi,
-                                              const float4* __restrict__ pos,
-                                              const uint32_t* __restrict__ start,
-                                              const uint32_t* __restrict__ nmask,
-                                              const uint32_t* __restrict__ nn, F&& f) {
-  const bool ovf = (nn[i] & UBP_NN_OVF) != 0u;
+__device__ __forceinline__ void walk27r(const DevParams& P, uint32_t i, float4 pi,
+                                        const float4* __restrict__ pos,
+                                        const uint32_t* __restrict__ start, F&& f) {
   uint32_t cx, cy, cz;
   pos_to_cell(P, pi.x, pi.y, pi.z, cx, cy, cz);
   Nbhd N;
   nbhd(P, cx, cy, cz, N);
-  const uint32_t* m = nmask + i;
 #pragma unroll 1
   for (int k = 0; k < 27; ++k) {
     const int dz = k / 9, dy = (k / 3) % 3, dx = k % 3;
     const uint32_t sx = sel3(N.sx, dx), sy = sel3(N.sy, dy), sz = sel3(N.sz, dz);
-    uint32_t mask0 = 0u;
-    if (!ovf) {
-      mask0 = m[size_t(k) * P.n];
-      if (mask0 == 0u) continue;
-    } else if (sx == 0xffffffffu || sy == 0xffffffffu || sz == 0xffffffffu) {
-      continue;
-    }
+    if (sx == 0xffffffffu || sy == 0xffffffffu || sz == 0xffffffffu) continue;
     const uint32_t code = sx | sy | sz;
-    const uint32_t b = start[code];
-    const uint32_t e = ovf ? start[code + 1u] : b + 1u;
-    for (uint32_t c0 = b; c0 < e; c0 += 32u) {
-      uint32_t mask = mask0;
-      if (ovf) {
-        const uint32_t ce = (e - c0 > 32u) ? c0 + 32u : e;
-        for (uint32_t s = c0; s < ce; ++s) {
-          if (s == i) continue;
-          float4 q = pos[s];
-          float ddx = pi.x - q.x;
-          float ddy = pi.y - q.y;
-          float ddz = pi.z - q.z;
-          float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-          if (d2 <= P.rsq) mask |= 1u << (s - c0);
-        }
-      }
-      while (mask) {
-        const uint32_t bit = uint32_t(__builtin_ctz(mask));
-        mask &= mask - 1u;
-        const uint32_t j = c0 + bit;
-        f(j, pos[j]);
-      }
+    const uint32_t b = start[code], e = start[code + 1u];
+    for (uint32_t s = b; s < e; ++s) {
+      if (s == i) continue;
+      float4 q = pos[s];
+      float ddx = pi.x - q.x;
+      float ddy = pi.y - q.y;
+      float ddz = pi.z - q.z;
+      float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+      if (d2 <= P.rsq) f(s, q);
     }
   }
 }
 
-// Cell walk once per step; the masks are reused by density, forces and the
-// next step's XSPH, which all see the same positions.
+template <class F>
+__device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, float4 pi,
+                                              const float4* __restrict__ pos,
+                                              const uint32_t* __restrict__ start,
+                                              const uint32_t* __restrict__ nbr,
+                                              const uint32_t* __restrict__ nn, F&& f) {
+  const uint32_t c = nn[i];
+  if (c <= P.K) {
+    const uint4* row = reinterpret_cast<const uint4*>(nbr + size_t(i) * P.K);
+    for (uint32_t t = 0; t < c; t += 4u) {
+      const uint4 q4 = row[t >> 2];
+      f(q4.x, pos[q4.x]);
+      if (t + 1u < c) f(q4.y, pos[q4.y]);
+      if (t + 2u < c) f(q4.z, pos[q4.z]);
+      if (t + 3u < c) f(q4.w, pos[q4.w]);
+    }
+  } else {
+    walk27r(P, i, pi, pos, start, f);
+  }
+}
+
 __global__ UBP_LB void k_neigh(DevParams P, const float4* __restrict__ pos,
-                               const uint32_t* __restrict__ start, uint32_t* __restrict__ nmask,
+                               const uint32_t* __restrict__ start, uint32_t* __restrict__ nbr,
                                uint32_t* __restrict__ nn) {
   uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   uint32_t i = blk * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   const float4 pi = pos[i];
-  uint32_t cx, cy, cz;
-  pos_to_cell(P, pi.x, pi.y, pi.z, cx, cy, cz);
-  Nbhd N;
-  nbhd(P, cx, cy, cz, N);
-  uint32_t c = 0, ovf = 0;
-  uint32_t* m = nmask + i;
-#pragma unroll 1
-  for (int k = 0; k < 27; ++k) {
-    const int dz = k / 9, dy = (k / 3) % 3, dx = k % 3;
-    const uint32_t sx = sel3(N.sx, dx), sy = sel3(N.sy, dy), sz = sel3(N.sz, dz);
-    uint32_t mask = 0u;
-    if (sx != 0xffffffffu && sy != 0xffffffffu && sz != 0xffffffffu) {
-      const uint32_t code = sx | sy | sz;
-      const uint32_t b = start[code], e = start[code + 1u];
-      if (e - b > 32u) ovf = UBP_NN_OVF;
-      for (uint32_t s = b; s < e; ++s) {
-        if (s == i) continue;
-        float4 q = pos[s];
-        float ddx = pi.x - q.x;
-        float ddy = pi.y - q.y;
-        float ddz = pi.z - q.z;
-        float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-        if (d2 <= P.rsq) {
-          ++c;
-          if (s - b < 32u) mask |= 1u << (s - b);
-        }
-      }
-    }
-    m[size_t(k) * P.n] = mask;
-  }
-  nn[i] = c | ovf;
+  uint32_t c = 0;
+  uint32_t* row = nbr + size_t(i) * P.K;
+  walk27r(P, i, pi, pos, start, [&](uint32_t j, float4) {
+    if (c < P.K) row[c] = j;
+    ++c;
+  });
+  nn[i] = c;
 }
 
 // Density summation + delta-SPH (old density) + SDF wall volume + Tait EOS.
@@ -330,7 +266,7 @@ __global__ UBP_LB void k_neigh(DevParams P, const float4* __restrict__ pos,
 __global__ UBP_LB void k_density(DevParams P, const float4* __restrict__ pos,
                                  const float2* __restrict__ rho_g, const float4* __restrict__ wall,
                                  const uint8_t* __restrict__ ft, const uint32_t* __restrict__ start,
-                                 const uint32_t* __restrict__ nmask, const uint32_t* __restrict__ nn,
+                                 const uint32_t* __restrict__ nbr, const uint32_t* __restrict__ nn,
                                  const float* __restrict__ dtp, float* __restrict__ rho_n,
                                  float* __restrict__ prs, float* __restrict__ prho2) {
   uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
@@ -342,7 +278,7 @@ __global__ UBP_LB void k_density(DevParams P, const float4* __restrict__ pos,
   Kahan rho, dif;
   rho.init(pi.w * P.norm_h3);  // m_i W(0, h)
   dif.init(0.0f);
-  for_neighbors(P, i, pi, pos, start, nmask, nn, [&](uint32_t j, float4 q) {
+  for_neighbors(P, i, pi, pos, start, nbr, nn, [&](uint32_t j, float4 q) {
     float dx = pi.x - q.x;
     float dy = pi.y - q.y;
     float dz = pi.z - q.z;
@@ -393,7 +329,7 @@ __global__ UBP_LB void k_forces(DevParams P, const float4* __restrict__ pos,
                                 const float4* __restrict__ vel, const float* __restrict__ rho_n,
                                 const float* __restrict__ prs, const float* __restrict__ prho2,
                                 const float4* __restrict__ wall, const uint8_t* __restrict__ ft,
-                                const uint32_t* __restrict__ start, const uint32_t* __restrict__ nmask,
+                                const uint32_t* __restrict__ start, const uint32_t* __restrict__ nbr,
                                 const uint32_t* __restrict__ nn, float4* __restrict__ acc) {
   uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   uint32_t i = blk * blockDim.x + threadIdx.x;
@@ -406,7 +342,7 @@ __global__ UBP_LB void k_forces(DevParams P, const float4* __restrict__ pos,
   Kahan fx, fy, fz, vx, vy, vz;
   fx.init(0.0f); fy.init(0.0f); fz.init(0.0f);
   vx.init(0.0f); vy.init(0.0f); vz.init(0.0f);
-  for_neighbors(P, i, pi, pos, start, nmask, nn, [&](uint32_t j, float4 q) {
+  for_neighbors(P, i, pi, pos, start, nbr, nn, [&](uint32_t j, float4 q) {
     float dx = pi.x - q.x;
     float dy = pi.y - q.y;
     float dz = pi.z - q.z;
@@ -687,7 +623,7 @@ __global__ UBP_LB void k_cfl_finalize(DevParams P, const uint32_t* __restrict__ 
 // neighbour masks of the last density pass (same positions).
 __global__ UBP_LB void k_xsph(DevParams P, const float4* __restrict__ pos,
                               const float4* __restrict__ vel, const float* __restrict__ rho_n,
-                              const uint32_t* __restrict__ start, const uint32_t* __restrict__ nmask,
+                              const uint32_t* __restrict__ start, const uint32_t* __restrict__ nbr,
                               const uint32_t* __restrict__ nn, float4* __restrict__ dv) {
   uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   uint32_t i = blk * blockDim.x + threadIdx.x;
@@ -696,7 +632,7 @@ __global__ UBP_LB void k_xsph(DevParams P, const float4* __restrict__ pos,
   float4 vi = vel[i];
   const float rho_i = rho_n[i];
   float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-  for_neighbors(P, i, pi, pos, start, nmask, nn, [&](uint32_t j, float4 q) {
+  for_neighbors(P, i, pi, pos, start, nbr, nn, [&](uint32_t j, float4 q) {
     float dx = pi.x - q.x;
     float dy = pi.y - q.y;
     float dz = pi.z - q.z;
