@@ -40,7 +40,7 @@ ALG_BYTES = {
     "k_scan": 0,
     "k_scatter": 8,            # R key, W permutation
     "k_rank_gather": 64,       # SoA gather, R + W pos, vel, rho, m
-    "k_neigh": 12,             # R pos
+    "k_neigh+k_expand": 12,    # R pos (the masks and lists are not algorithmic)
     "k_density": 28,           # R pos 12, rho_old 4, m 4; W rho 4, P 4
     "k_forces": 48,            # R pos 12, vel 12, rho 4, P 4, m 4; W acc 12
     "k_kick2_cfl": 36,         # R vel 12, acc 12; W vel 12

@@ -140,7 +140,7 @@ int ubp_step_profiled(ubp_engine *e, float dt, ubp_step_profile *out);
  * handle's stream around every launch; kernel_ms[k] receives the summed
  * milliseconds of kernel k over the n steps, in the order:
  *   0 kick, 1 xsph + drift, 2 scan (3 launches), 3 scatter, 4 rank_gather,
- *   5 neigh, 6 density, 7 forces, 8 kick2_cfl.
+ *   5 neigh + expand, 6 density (+ adami), 7 forces, 8 kick2_cfl + cfl_finalize.
  * Synchronous. */
 #define UBP_NKERN 9
 int ubp_step_n_profiled(ubp_engine *e, int nsteps, double *kernel_ms);

@@ -85,7 +85,7 @@ EXPORTS = ("ubp_create", "ubp_destroy", "ubp_last_error", "ubp_gpu_available",
            "ubp_step_n_profiled", "ubp_calibration_copy")
 
 #: kernel order of ubp_step_n_profiled
-KERNELS = ("k_kick", "k_xsph+k_drift", "k_scan", "k_scatter", "k_rank_gather", "k_neigh",
+KERNELS = ("k_kick", "k_xsph+k_drift", "k_scan", "k_scatter", "k_rank_gather", "k_neigh+k_expand",
            "k_density", "k_forces", "k_kick2_cfl")
 
 

@@ -179,9 +179,9 @@ __device__ __forceinline__ bool sdf_sample(const DevParams& P, float px, float p
 // Wall record at a position: (n, d) with d = +inf when the wall cannot act
 // (outside the SDF grid, or d >= 2h so that no wall term is non-zero).  The
 // normal is the normalised central difference at +-cell/2 (geometry.rs:
-// 153-175); any sample leaving the grid disables the wall.  Out of line: it is
-// evaluated once or twice per particle, not per pair.
-__device__ __noinline__ float4 wall_at(const DevParams& P, float px, float py, float pz) {
+// 153-175); any sample leaving the grid disables the wall.  The sample loop
+// is rolled so the function stays small where it is inlined.
+__device__ __forceinline__ float4 wall_at(const DevParams& P, float px, float py, float pz) {
   const float inf = __builtin_inff();
   if (!P.sdf_on) return make_float4(0.0f, 0.0f, 0.0f, inf);
   float d;

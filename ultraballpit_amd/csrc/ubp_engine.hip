@@ -85,7 +85,7 @@ struct Engine {
   uint32_t* orig[2] = {nullptr, nullptr};
   int pp = 0;
   uint32_t *key = nullptr, *arr = nullptr, *cnt = nullptr, *start = nullptr, *bsum = nullptr;
-  uint32_t *stage_src = nullptr, *stage_orig = nullptr, *nbr = nullptr, *nn = nullptr;
+  uint32_t *stage_src = nullptr, *stage_orig = nullptr, *nbr = nullptr, *nn = nullptr, *nmask = nullptr;
   uint32_t* spread = nullptr;
   float* sdf = nullptr;
   float* dtp = nullptr;
@@ -135,7 +135,7 @@ struct Engine {
     dfree(rho_g); dfree(rho_n); dfree(prs); dfree(prho2);
     for (int k = 0; k < 2; ++k) { dfree(ft[k]); dfree(orig[k]); }
     dfree(key); dfree(arr); dfree(cnt); dfree(start); dfree(bsum); dfree(stage_src);
-    dfree(stage_orig); dfree(nbr); dfree(nn); dfree(spread); dfree(sdf); dfree(dtp); dfree(red);
+    dfree(stage_orig); dfree(nbr); dfree(nn); dfree(nmask); dfree(spread); dfree(sdf); dfree(dtp); dfree(red);
     dfree(pmax); dfree(psum); dfree(dbg);
     dfree(bpos); dfree(bprs); dfree(bl_start); dfree(bl); dfree(bc_start); dfree(bc);
     dfree(ck_pos); dfree(ck_vel); dfree(ck_acc); dfree(ck_wall); dfree(ck_rho); dfree(ck_prs);
@@ -368,7 +368,18 @@ struct Engine {
 
   void alloc_nbr() {
     dfree(nbr);
-    dalloc(&nbr, size_t(P.K) * n);  // neighbour lists, one row of K per particle
+    dalloc(&nbr, size_t(P.K) * n);  // flat neighbour lists, column-major [K][N]
+    if (!nmask) dalloc(&nmask, size_t(27) * n);  // per-cell masks [27][N]
+  }
+
+  // neighbour masks of the current positions, then the flat lists
+  void build_neighbors() {
+    launch(k_neigh, grid_of(n), P, pos, start, nmask, nn);
+    if (n) {
+      hipLaunchKernelGGL(k_expand, dim3(uint32_t((n + 63) / 64)), dim3(64), 0, st, P, (const float4*)pos,
+                         (const uint32_t*)start, (const uint32_t*)nmask, (const uint32_t*)nn, nbr);
+      LAUNCHCHK("k_expand");
+    }
   }
 
   void invalidate_graph() {
@@ -408,11 +419,11 @@ struct Engine {
       block = bs;
     }
     if (K && K != P.K) {
-      if (K % 4 != 0 || K < 4 || K > 1024) throw Error(UBP_EINVAL, "neighbour capacity must be a multiple of 4 in [4, 1024]");
+      if (K < 1 || K > UBP_EXPAND_K) throw Error(UBP_EINVAL, "neighbour capacity must be in [1, 128]");
       HIPCHK(hipStreamSynchronize(st));
       P.K = K;
       alloc_nbr();
-      if (!needs_init) launch(k_neigh, grid_of(n), P, pos, start, nbr, nn);  // rebuild lists
+      if (!needs_init) build_neighbors();
     }
     invalidate_graph();
   }
@@ -452,7 +463,7 @@ struct Engine {
   void density_forces(hipEvent_t* E = nullptr) {
     const uint32_t g = grid_of(n);
     rec(E, 5);
-    launch(k_neigh, g, P, pos, start, nbr, nn);
+    build_neighbors();
     rec(E, 6);
     launch(k_density, g, P, pos, rho_g, wall, ft[pp], start, nbr, nn, dtp, rho_n, prs, prho2);
     if (P.nb) launch(k_adami, grid_of(P.nb), P, pos, rho_n, prs, orig[pp], start, bprs);
@@ -725,7 +736,7 @@ struct Engine {
     cp(ft[pp], ck_ft, n);
     cp(orig[pp], ck_orig, n * 4);
     cp(start, ck_start, (size_t(P.ncodes) + 1) * 4);
-    if (!needs_init) launch(k_neigh, grid_of(n), P, pos, start, nbr, nn);  // lists of the restored grid
+    if (!needs_init) build_neighbors();  // neighbour lists of the restored state
     has_ckpt = false;
     dt_valid = false;
     invalidate_graph();

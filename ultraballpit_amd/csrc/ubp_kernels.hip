@@ -167,11 +167,17 @@ __global__ UBP_LB void k_rank_gather(DevParams P, const uint32_t* __restrict__ s
   orig_dst[t] = me;
 }
 
-// Neighbour lists: row i holds the accepted neighbour slots of particle i in
-// the reference visiting order (cells dz, dy, dx; slots ascending), up to K
-// entries: nbr[i * K + t].  A lane fills its own row front to back, so the
-// partial-line stores of a wavefront merge in L2 into whole lines.  nn[i] is
-// the accepted count; a count above K makes the passes re-walk the cells.
+// Neighbour structure, built in two passes each step:
+//  k_neigh  walks the 27 cells once (reference order: dz, dy, dx; slots
+//           ascending) and stores one 32-bit mask of accepted slots per cell,
+//           nmask[k * N + i] (column-major: every store coalesced);
+//  k_expand turns the masks into flat lists nbr[t * N + i] through an LDS
+//           tile per wavefront, so the list rows are also stored coalesced.
+// The density, force and XSPH passes read the flat lists (one coalesced
+// 4-byte load per neighbour).  nn[i] is the accepted count; bit 31 flags a
+// cell with more than 32 particles, and a count above K leaves the list
+// empty: such particles re-walk the cells in the passes.
+#define UBP_NN_OVF 0x80000000u
 
 __device__ __forceinline__ uint32_t sel3(const uint32_t v[3], int d) {
   return d == 0 ? v[0] : (d == 1 ? v[1] : v[2]);
@@ -230,14 +236,14 @@ __device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, fl
                                               const uint32_t* __restrict__ nbr,
                                               const uint32_t* __restrict__ nn, F&& f) {
   const uint32_t c = nn[i];
-  if (c <= P.K) {
-    const uint4* row = reinterpret_cast<const uint4*>(nbr + size_t(i) * P.K);
-    for (uint32_t t = 0; t < c; t += 4u) {
-      const uint4 q4 = row[t >> 2];
-      f(q4.x, pos[q4.x]);
-      if (t + 1u < c) f(q4.y, pos[q4.y]);
-      if (t + 2u < c) f(q4.z, pos[q4.z]);
-      if (t + 3u < c) f(q4.w, pos[q4.w]);
+  if (c <= P.K) {  // no overflow bit and the list fits
+    const uint32_t* l = nbr + i;
+    const size_t stride = P.n;
+    uint32_t jn = (c > 0u) ? l[0] : 0u;
+    for (uint32_t t = 0; t < c; ++t) {
+      const uint32_t j = jn;
+      if (t + 1u < c) jn = l[size_t(t + 1u) * stride];  // next index in flight
+      f(j, pos[j]);
     }
   } else {
     walk27r(P, i, pi, pos, start, f);
@@ -245,19 +251,94 @@ __device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, fl
 }
 
 __global__ UBP_LB void k_neigh(DevParams P, const float4* __restrict__ pos,
-                               const uint32_t* __restrict__ start, uint32_t* __restrict__ nbr,
+                               const uint32_t* __restrict__ start, uint32_t* __restrict__ nmask,
                                uint32_t* __restrict__ nn) {
   uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   uint32_t i = blk * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   const float4 pi = pos[i];
+  uint32_t cx, cy, cz;
+  pos_to_cell(P, pi.x, pi.y, pi.z, cx, cy, cz);
+  Nbhd N;
+  nbhd(P, cx, cy, cz, N);
+  uint32_t c = 0, ovf = 0;
+  uint32_t* m = nmask + i;
+#pragma unroll 1
+  for (int k = 0; k < 27; ++k) {
+    const int dz = k / 9, dy = (k / 3) % 3, dx = k % 3;
+    const uint32_t sx = sel3(N.sx, dx), sy = sel3(N.sy, dy), sz = sel3(N.sz, dz);
+    uint32_t mask = 0u;
+    if (sx != 0xffffffffu && sy != 0xffffffffu && sz != 0xffffffffu) {
+      const uint32_t code = sx | sy | sz;
+      const uint32_t b = start[code], e = start[code + 1u];
+      if (e - b > 32u) ovf = UBP_NN_OVF;
+      for (uint32_t s = b; s < e; ++s) {
+        if (s == i) continue;
+        float4 q = pos[s];
+        float ddx = pi.x - q.x;
+        float ddy = pi.y - q.y;
+        float ddz = pi.z - q.z;
+        float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        if (d2 <= P.rsq) {
+          ++c;
+          if (s - b < 32u) mask |= 1u << (s - b);
+        }
+      }
+    }
+    m[size_t(k) * P.n] = mask;
+  }
+  nn[i] = c | ovf;
+}
+
+// Masks -> flat lists.  One wavefront per 64-thread workgroup; each lane
+// decodes its masks in order into an LDS column, then the tile is stored row
+// by row (t = 0, 1, ...), 64 consecutive slots per row.
+#define UBP_EXPAND_K 128
+__global__ __launch_bounds__(64) void k_expand(DevParams P, const float4* __restrict__ pos,
+                                               const uint32_t* __restrict__ start,
+                                               const uint32_t* __restrict__ nmask,
+                                               const uint32_t* __restrict__ nn,
+                                               uint32_t* __restrict__ nbr) {
+  __shared__ uint32_t tile[UBP_EXPAND_K][64];
+  const uint32_t lane = threadIdx.x;
+  const uint32_t i = blockIdx.x * 64u + lane;
   uint32_t c = 0;
-  uint32_t* row = nbr + size_t(i) * P.K;
-  walk27r(P, i, pi, pos, start, [&](uint32_t j, float4) {
-    if (c < P.K) row[c] = j;
-    ++c;
-  });
-  nn[i] = c;
+  if (i < P.n) {
+    const uint32_t nv = nn[i];
+    if (nv <= P.K) {
+      const float4 pi = pos[i];
+      uint32_t cx, cy, cz;
+      pos_to_cell(P, pi.x, pi.y, pi.z, cx, cy, cz);
+      Nbhd N;
+      nbhd(P, cx, cy, cz, N);
+      const uint32_t* m = nmask + i;
+#pragma unroll 1
+      for (int k = 0; k < 27; ++k) {
+        uint32_t mask = m[size_t(k) * P.n];
+        if (mask == 0u) continue;
+        const int dz = k / 9, dy = (k / 3) % 3, dx = k % 3;
+        const uint32_t b = start[sel3(N.sx, dx) | sel3(N.sy, dy) | sel3(N.sz, dz)];
+        while (mask) {
+          const uint32_t bit = uint32_t(__builtin_ctz(mask));
+          mask &= mask - 1u;
+          tile[c][lane] = b + bit;
+          ++c;
+        }
+      }
+    }
+  }
+  // longest list in the wavefront
+  uint32_t tmax = c;
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t t = __shfl_xor(tmax, o, 64);
+    tmax = tmax > t ? tmax : t;
+  }
+  __syncthreads();
+  if (i < P.n) {
+    uint32_t* l = nbr + i;
+    for (uint32_t t = 0; t < tmax; ++t)
+      if (t < c) l[size_t(t) * P.n] = tile[t][lane];
+  }
 }
 
 // Density summation + delta-SPH (old density) + SDF wall volume + Tait EOS.
@@ -668,8 +749,14 @@ __global__ UBP_LB void k_drift(DevParams P, const float4* __restrict__ pos,
   float y = pi.y + (vi.y + d.y) * dt;
   float z = pi.z + (vi.z + d.z) * dt;
   float vx = vi.x, vy = vi.y, vz = vi.z;
-  if (P.sdf_on) {
-    float4 w = wall_at(P, x, y, z);
+  // Wall record at the drifted position; it is also the record of the final
+  // position unless the SDF projection or the domain clamp moved the particle,
+  // in which case it is evaluated once more (one call site, inlined).
+  float4 w = make_float4(0.0f, 0.0f, 0.0f, __builtin_inff());
+  for (int pass = 0; pass < 2; ++pass) {
+    w = wall_at(P, x, y, z);
+    if (pass == 1) break;
+    const float x0 = x, y0 = y, z0 = z;
     if (w.w != __builtin_inff() && w.w < 0.0f) {
       float dd = w.w;
       x -= dd * w.x; y -= dd * w.y; z -= dd * w.z;
@@ -679,17 +766,21 @@ __global__ UBP_LB void k_drift(DevParams P, const float4* __restrict__ pos,
         vx -= k * w.x; vy -= k * w.y; vz -= k * w.z;
       }
     }
+    const float rest = 0.2f;
+    if (x < P.dmin[0]) { x = P.dmin[0]; if (vx < 0.0f) vx = -rest * vx; }
+    if (x > P.dmax[0]) { x = P.dmax[0]; if (vx > 0.0f) vx = -rest * vx; }
+    if (y < P.dmin[1]) { y = P.dmin[1]; if (vy < 0.0f) vy = -rest * vy; }
+    if (y > P.dmax[1]) { y = P.dmax[1]; if (vy > 0.0f) vy = -rest * vy; }
+    if (z < P.dmin[2]) { z = P.dmin[2]; if (vz < 0.0f) vz = -rest * vz; }
+    if (z > P.dmax[2]) { z = P.dmax[2]; if (vz > 0.0f) vz = -rest * vz; }
+    // bitwise position test: a move by zero keeps the record
+    if (__float_as_uint(x) == __float_as_uint(x0) && __float_as_uint(y) == __float_as_uint(y0) &&
+        __float_as_uint(z) == __float_as_uint(z0))
+      break;
   }
-  const float rest = 0.2f;
-  if (x < P.dmin[0]) { x = P.dmin[0]; if (vx < 0.0f) vx = -rest * vx; }
-  if (x > P.dmax[0]) { x = P.dmax[0]; if (vx > 0.0f) vx = -rest * vx; }
-  if (y < P.dmin[1]) { y = P.dmin[1]; if (vy < 0.0f) vy = -rest * vy; }
-  if (y > P.dmax[1]) { y = P.dmax[1]; if (vy > 0.0f) vy = -rest * vy; }
-  if (z < P.dmin[2]) { z = P.dmin[2]; if (vz < 0.0f) vz = -rest * vz; }
-  if (z > P.dmax[2]) { z = P.dmax[2]; if (vz > 0.0f) vz = -rest * vz; }
   pos_t[i] = make_float4(x, y, z, pi.w);
   vel_t[i] = make_float4(vx, vy, vz, 0.0f);
-  wall_t[i] = wall_at(P, x, y, z);
+  wall_t[i] = w;
   uint32_t ccx, ccy, ccz;
   pos_to_cell(P, x, y, z, ccx, ccy, ccz);
   uint32_t code = morton(P, ccx, ccy, ccz);
