@@ -64,23 +64,26 @@ __device__ __forceinline__ float div_h(float r, const DevParams& P) {
 }
 
 // ---- Wendland C2 (sph.rs:35-46, 68-85) ----
+// Branch-free forms: the value of the reference expression is computed and
+// the early-return zero is selected, so the bits are those of the branched
+// code while the compiler can interleave independent neighbours.
 __device__ __forceinline__ float wendland_q(float q, const DevParams& P) {
-  if (q >= 2.0f) return 0.0f;
   float omq = 1.0f - 0.5f * q;
   float t = omq * omq;
   float t4 = t * t;
-  return P.norm_h3 * t4 * (1.0f + 2.0f * q);
+  float w = P.norm_h3 * t4 * (1.0f + 2.0f * q);
+  return (q >= 2.0f) ? 0.0f : w;
 }
 __device__ __forceinline__ void wgrad_q(float dx, float dy, float dz, float r, float q,
                                         const DevParams& P, float& gx, float& gy, float& gz) {
-  if (q >= 2.0f || r < 1.0e-12f) { gx = 0.0f; gy = 0.0f; gz = 0.0f; return; }
+  const bool zero = (q >= 2.0f) || (r < 1.0e-12f);
   float omq = 1.0f - 0.5f * q;
   float t3 = omq * omq * omq;
   float dw = P.norm_h4 * (-5.0f * q) * t3;
   float inv_r = 1.0f / r;
-  gx = dw * dx * inv_r;
-  gy = dw * dy * inv_r;
-  gz = dw * dz * inv_r;
+  gx = zero ? 0.0f : dw * dx * inv_r;
+  gy = zero ? 0.0f : dw * dy * inv_r;
+  gz = zero ? 0.0f : dw * dz * inv_r;
 }
 
 // Compensated (Kahan) accumulator: t = s + y; c = (t - s) - y.
@@ -92,6 +95,14 @@ struct Kahan {
     float t = s + y;
     c = (t - s) - y;
     s = t;
+  }
+  // add(v) when `on`, otherwise leave the sum untouched (branch-free)
+  __device__ __forceinline__ void add_if(bool on, float v) {
+    float y = v - c;
+    float t = s + y;
+    float cn = (t - s) - y;
+    s = on ? t : s;
+    c = on ? cn : c;
   }
 };
 

@@ -239,11 +239,19 @@ __device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, fl
   if (c <= P.K) {  // no overflow bit and the list fits
     const uint32_t* l = nbr + i;
     const size_t stride = P.n;
-    uint32_t jn = (c > 0u) ? l[0] : 0u;
-    for (uint32_t t = 0; t < c; ++t) {
-      const uint32_t j = jn;
-      if (t + 1u < c) jn = l[size_t(t + 1u) * stride];  // next index in flight
-      f(j, pos[j]);
+    uint32_t t = 0;
+    // two neighbours per trip: both gathers in flight, bodies in list order
+    for (; t + 1u < c; t += 2u) {
+      const uint32_t j0 = l[size_t(t) * stride];
+      const uint32_t j1 = l[size_t(t + 1u) * stride];
+      const float4 q0 = pos[j0];
+      const float4 q1 = pos[j1];
+      f(j0, q0);
+      f(j1, q1);
+    }
+    if (t < c) {
+      const uint32_t j0 = l[size_t(t) * stride];
+      f(j0, pos[j0]);
     }
   } else {
     walk27r(P, i, pi, pos, start, f);
@@ -312,12 +320,19 @@ __global__ __launch_bounds__(64) void k_expand(DevParams P, const float4* __rest
       Nbhd N;
       nbhd(P, cx, cy, cz, N);
       const uint32_t* m = nmask + i;
-#pragma unroll 1
+      uint32_t mk[27], bk[27];
+#pragma unroll
+      for (int k = 0; k < 27; ++k) mk[k] = m[size_t(k) * P.n];  // 27 loads in flight
+#pragma unroll
       for (int k = 0; k < 27; ++k) {
-        uint32_t mask = m[size_t(k) * P.n];
-        if (mask == 0u) continue;
         const int dz = k / 9, dy = (k / 3) % 3, dx = k % 3;
-        const uint32_t b = start[sel3(N.sx, dx) | sel3(N.sy, dy) | sel3(N.sz, dz)];
+        const uint32_t code = N.sx[dx] | N.sy[dy] | N.sz[dz];
+        bk[k] = (mk[k] != 0u) ? start[code] : 0u;  // masks are 0 off the grid
+      }
+#pragma unroll
+      for (int k = 0; k < 27; ++k) {
+        uint32_t mask = mk[k];
+        const uint32_t b = bk[k];
         while (mask) {
           const uint32_t bit = uint32_t(__builtin_ctz(mask));
           mask &= mask - 1u;
@@ -438,14 +453,13 @@ __global__ UBP_LB void k_forces(DevParams P, const float4* __restrict__ pos,
     float dvy = vi.y - vj.y;
     float dvz = vi.z - vj.z;
     float vr = dvx * dx + dvy * dy + dvz * dz;
-    if (vr < 0.0f) {
-      float r_sq = r * r;
-      float mu = P.h * vr / (r_sq + P.eta_sq);
-      float rho_avg = 0.5f * (rho_i + rho_n[j]);
-      float pij = (P.nac * mu + P.beta * mu * mu) / rho_avg;
-      float vf = -q.w * pij;
-      vx.add(vf * gx); vy.add(vf * gy); vz.add(vf * gz);
-    }
+    const bool app = vr < 0.0f;  // approaching pairs only (sph.rs:467-468)
+    float r_sq = r * r;
+    float mu = P.h * vr / (r_sq + P.eta_sq);
+    float rho_avg = 0.5f * (rho_i + rho_n[j]);
+    float pij = (P.nac * mu + P.beta * mu * mu) / rho_avg;
+    float vf = -q.w * pij;
+    vx.add_if(app, vf * gx); vy.add_if(app, vf * gy); vz.add_if(app, vf * gz);
   });
   uint32_t bk0 = 0, bk1 = 0;
   if (P.nb) {  // boundary pressure (sph.rs:376-408), ascending boundary index
