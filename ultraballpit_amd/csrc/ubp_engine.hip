@@ -79,8 +79,8 @@ struct Engine {
 
   float4 *pos = nullptr, *pos_t = nullptr, *vel = nullptr, *vel_t = nullptr, *acc = nullptr;
   float4 *wall = nullptr, *wall_t = nullptr, *dv = nullptr;
-  float2* rho_g = nullptr;
-  float *rho_n = nullptr, *prs = nullptr, *prho2 = nullptr;
+  float *rho_n = nullptr, *prs = nullptr;
+  float4 *dr = nullptr, *fr = nullptr, *xr = nullptr;  // per-pass neighbour records
   uint8_t* ft[2] = {nullptr, nullptr};
   uint32_t* orig[2] = {nullptr, nullptr};
   int pp = 0;
@@ -103,8 +103,7 @@ struct Engine {
   bool ck_needs_init = true;
   int ck_pp = 0;
   float4 *ck_pos = nullptr, *ck_vel = nullptr, *ck_acc = nullptr, *ck_wall = nullptr;
-  float *ck_rho = nullptr, *ck_prs = nullptr, *ck_prho2 = nullptr;
-  float2* ck_rho_g = nullptr;
+  float *ck_rho = nullptr, *ck_prs = nullptr;
   uint8_t* ck_ft = nullptr;
   uint32_t *ck_orig = nullptr, *ck_start = nullptr;
 
@@ -132,14 +131,14 @@ struct Engine {
     if (gexec) { (void)hipGraphExecDestroy(gexec); gexec = nullptr; }
     if (st) { (void)hipStreamSynchronize(st); }
     dfree(pos); dfree(pos_t); dfree(vel); dfree(vel_t); dfree(acc); dfree(wall); dfree(wall_t); dfree(dv);
-    dfree(rho_g); dfree(rho_n); dfree(prs); dfree(prho2);
+    dfree(rho_n); dfree(prs); dfree(dr); dfree(fr); dfree(xr);
     for (int k = 0; k < 2; ++k) { dfree(ft[k]); dfree(orig[k]); }
     dfree(key); dfree(arr); dfree(cnt); dfree(start); dfree(bsum); dfree(stage_src);
     dfree(stage_orig); dfree(nbr); dfree(nn); dfree(nmask); dfree(spread); dfree(sdf); dfree(dtp); dfree(red);
     dfree(pmax); dfree(psum); dfree(dbg);
     dfree(bpos); dfree(bprs); dfree(bl_start); dfree(bl); dfree(bc_start); dfree(bc);
     dfree(ck_pos); dfree(ck_vel); dfree(ck_acc); dfree(ck_wall); dfree(ck_rho); dfree(ck_prs);
-    dfree(ck_prho2); dfree(ck_rho_g); dfree(ck_ft); dfree(ck_orig); dfree(ck_start);
+    dfree(ck_ft); dfree(ck_orig); dfree(ck_start);
     if (h_stage) { (void)hipHostFree(h_stage); h_stage = nullptr; }
     if (st) { (void)hipStreamDestroy(st); st = nullptr; }
   }
@@ -248,7 +247,8 @@ struct Engine {
     // device state
     dalloc(&pos, n); dalloc(&pos_t, n); dalloc(&vel, n); dalloc(&vel_t, n); dalloc(&acc, n);
     dalloc(&wall, n); dalloc(&wall_t, n); dalloc(&dv, n);
-    dalloc(&rho_g, n); dalloc(&rho_n, n); dalloc(&prs, n); dalloc(&prho2, n);
+    dalloc(&rho_n, n); dalloc(&prs, n);
+    dalloc(&dr, 2 * n); dalloc(&fr, 4 * n); dalloc(&xr, 2 * n);
     for (int k = 0; k < 2; ++k) { dalloc(&ft[k], n); dalloc(&orig[k], n); }
     dalloc(&key, n); dalloc(&arr, n); dalloc(&stage_src, n); dalloc(&stage_orig, n); dalloc(&nn, n);
     dalloc(&cnt, size_t(P.ncodes) + 1); dalloc(&start, size_t(P.ncodes) + 1);
@@ -456,7 +456,7 @@ struct Engine {
     launch(k_scatter, g, P, key, arr, start, orig[pp], stage_src, stage_orig);
     rec(E, 4);
     launch(k_rank_gather, g, P, stage_src, stage_orig, key, start, pos_t, vel_t, rho_n, wall_t, ft[pp],
-           pos, vel, rho_g, wall, ft[pp ^ 1], orig[pp ^ 1]);
+           pos, vel, dr, wall, ft[pp ^ 1], orig[pp ^ 1]);
     pp ^= 1;
   }
 
@@ -465,10 +465,10 @@ struct Engine {
     rec(E, 5);
     build_neighbors();
     rec(E, 6);
-    launch(k_density, g, P, pos, rho_g, wall, ft[pp], start, nbr, nn, dtp, rho_n, prs, prho2);
+    launch(k_density, g, P, dr, vel, wall, ft[pp], start, nbr, nn, dtp, rho_n, prs, fr);
     if (P.nb) launch(k_adami, grid_of(P.nb), P, pos, rho_n, prs, orig[pp], start, bprs);
     rec(E, 7);
-    launch(k_forces, g, P, pos, vel, rho_n, prs, prho2, wall, ft[pp], start, nbr, nn, acc);
+    launch(k_forces, g, P, fr, wall, ft[pp], start, nbr, nn, acc);
     rec(E, 8);
   }
 
@@ -484,9 +484,9 @@ struct Engine {
   void step_body(int do_cfl, hipEvent_t* E = nullptr) {
     const uint32_t g = grid_of(n);
     rec(E, 0);
-    launch(k_kick, g, P, vel, acc, dtp);
+    launch(k_kick, g, P, vel, acc, pos, rho_n, dtp, xr);
     rec(E, 1);
-    launch(k_xsph, g, P, pos, vel, rho_n, start, nbr, nn, dv);
+    launch(k_xsph, g, P, xr, start, nbr, nn, dv);
     launch(k_drift, g, P, pos, vel, dv, dtp, pos_t, vel_t, wall_t, key, arr, cnt);
     rec(E, 2);
     bin(E);
@@ -699,7 +699,7 @@ struct Engine {
   void save_ckpt() {
     if (!ck_pos) {
       dalloc(&ck_pos, n); dalloc(&ck_vel, n); dalloc(&ck_acc, n); dalloc(&ck_wall, n);
-      dalloc(&ck_rho, n); dalloc(&ck_prs, n); dalloc(&ck_prho2, n); dalloc(&ck_rho_g, n);
+      dalloc(&ck_rho, n); dalloc(&ck_prs, n);
       dalloc(&ck_ft, n); dalloc(&ck_orig, n); dalloc(&ck_start, size_t(P.ncodes) + 1);
     }
     auto cp = [&](void* d, const void* s, size_t b) { if (b) HIPCHK(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, st)); };
@@ -710,8 +710,6 @@ struct Engine {
     cp(ck_wall, wall, n * 16);
     cp(ck_rho, rho_n, n * 4);
     cp(ck_prs, prs, n * 4);
-    cp(ck_prho2, prho2, n * 4);
-    cp(ck_rho_g, rho_g, n * 8);
     cp(ck_ft, ft[pp], n);
     cp(ck_orig, orig[pp], n * 4);
     cp(ck_start, start, (size_t(P.ncodes) + 1) * 4);
@@ -731,8 +729,6 @@ struct Engine {
     cp(wall, ck_wall, n * 16);
     cp(rho_n, ck_rho, n * 4);
     cp(prs, ck_prs, n * 4);
-    cp(prho2, ck_prho2, n * 4);
-    cp(rho_g, ck_rho_g, n * 8);
     cp(ft[pp], ck_ft, n);
     cp(orig[pp], ck_orig, n * 4);
     cp(start, ck_start, (size_t(P.ncodes) + 1) * 4);

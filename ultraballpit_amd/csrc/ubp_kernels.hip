@@ -3,7 +3,10 @@
 // Data layout in HBM (slot order = sorted by (Morton cell key, original id)):
 //   pos   float4 (x, y, z, m)      vel  float4 (vx, vy, vz, 0)
 //   acc   float4 (ax, ay, az, 0)   wall float4 (nx, ny, nz, d) SDF record
-//   rho_g old density (gathered)   rho_n new density, prs, prho2 = P/rho^2
+//   rho_n new density, prs         records (one cache line per neighbour):
+//   dr (x y z m)(rho_old m/rho_old) for density, written by the gather
+//   fr (x y z m)(vx vy vz rho)(P/rho^2 P) for forces, written by density
+//   xr (x y z m)(vx vy vz rho) for XSPH, written by the first half-kick
 //   ft u8, orig u32 (ping-pong)    nbr u32 [N][K] neighbour lists (row per particle)
 // One step (paths relative to the reference repository; CPU step
 // backend/crates/kernel/src/lib.rs:287-335):
@@ -142,7 +145,7 @@ __global__ UBP_LB void k_rank_gather(DevParams P, const uint32_t* __restrict__ s
                                      const float* __restrict__ rho_src, const float4* __restrict__ wall_t,
                                      const uint8_t* __restrict__ ft_src,
                                      float4* __restrict__ pos, float4* __restrict__ vel,
-                                     float2* __restrict__ rho_g, float4* __restrict__ wall,
+                                     float4* __restrict__ dr, float4* __restrict__ wall,
                                      uint8_t* __restrict__ ft_dst, uint32_t* __restrict__ orig_dst) {
   uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= P.n) return;
@@ -161,7 +164,8 @@ __global__ UBP_LB void k_rank_gather(DevParams P, const uint32_t* __restrict__ s
   if (P.diff_floor > 0.0f && !(rd >= P.diff_floor)) rd = P.diff_floor;
   pos[t] = p;
   vel[t] = vel_t[s];
-  rho_g[t] = make_float2(rho, p.w / rd);
+  dr[2u * t] = p;                                          // density record:
+  dr[2u * t + 1u] = make_float4(rho, p.w / rd, 0.0f, 0.0f);  // (x y z m)(rho_old m/rho_old)
   wall[t] = wall_t[s];
   ft_dst[t] = ft_src[s];
   orig_dst[t] = me;
@@ -198,40 +202,68 @@ __device__ __forceinline__ void nbhd(const DevParams& P, uint32_t cx, uint32_t c
   }
 }
 
+// Squared gaps from p to the three cell slabs below / at / above its own cell
+// along one axis.  A cell's slab is [min + c cell, min + (c+1) cell]; the
+// outermost cells are open towards the outside (clamped particles land
+// there).  Particles binned into a cell lie in its slab up to rounding, so a
+// gap sum above (2h)^2 (1 + 1e-4) proves the cell holds no neighbour.
+__device__ __forceinline__ void slab_gaps(float p, float gmin, float cell, uint32_t c, uint32_t dim,
+                                          float g2[3]) {
+  const float lo = gmin + float(c) * cell;        // lower face of the own cell
+  const float hi = gmin + float(c + 1u) * cell;   // upper face
+  const float gl = (c > 0u && p > lo) ? p - lo : 0.0f;        // to the cell below
+  const float gu = (c + 1u < dim && p < hi) ? hi - p : 0.0f;  // to the cell above
+  g2[0] = gl * gl;
+  g2[1] = 0.0f;
+  g2[2] = gu * gu;
+}
+
 // Walk the 27 cells around particle i in the reference order (dz, dy, dx;
 // neighbor.rs:147-183), slots ascending (= original id ascending), skip i,
-// accept dist^2 <= (2h)^2; f(j, pos_j) for accepted neighbours.  Rolled: one
-// copy of the body.
-template <class F>
+// accept dist^2 <= (2h)^2; f(j, pos_j, rec_j) for accepted neighbours, where
+// positions are rec[STRIDE * slot].  Cells that provably hold no neighbour
+// are skipped (no change to the accepted sequence).  Rolled: one copy of the
+// body.
+template <int STRIDE, class F>
 __device__ __forceinline__ void walk27r(const DevParams& P, uint32_t i, float4 pi,
-                                        const float4* __restrict__ pos,
+                                        const float4* __restrict__ rec,
                                         const uint32_t* __restrict__ start, F&& f) {
   uint32_t cx, cy, cz;
   pos_to_cell(P, pi.x, pi.y, pi.z, cx, cy, cz);
   Nbhd N;
   nbhd(P, cx, cy, cz, N);
+  float gx2[3], gy2[3], gz2[3];
+  slab_gaps(pi.x, P.dmin[0], P.cell, cx, P.dims[0], gx2);
+  slab_gaps(pi.y, P.dmin[1], P.cell, cy, P.dims[1], gy2);
+  slab_gaps(pi.z, P.dmin[2], P.cell, cz, P.dims[2], gz2);
+  const float skip2 = P.rsq * 1.0001f;
 #pragma unroll 1
   for (int k = 0; k < 27; ++k) {
     const int dz = k / 9, dy = (k / 3) % 3, dx = k % 3;
     const uint32_t sx = sel3(N.sx, dx), sy = sel3(N.sy, dy), sz = sel3(N.sz, dz);
     if (sx == 0xffffffffu || sy == 0xffffffffu || sz == 0xffffffffu) continue;
+    const float gap2 = (dx == 0 ? gx2[0] : (dx == 1 ? gx2[1] : gx2[2])) +
+                       (dy == 0 ? gy2[0] : (dy == 1 ? gy2[1] : gy2[2])) +
+                       (dz == 0 ? gz2[0] : (dz == 1 ? gz2[1] : gz2[2]));
+    if (gap2 > skip2) continue;
     const uint32_t code = sx | sy | sz;
     const uint32_t b = start[code], e = start[code + 1u];
     for (uint32_t s = b; s < e; ++s) {
       if (s == i) continue;
-      float4 q = pos[s];
+      const float4* r = rec + size_t(STRIDE) * s;
+      float4 q = r[0];
       float ddx = pi.x - q.x;
       float ddy = pi.y - q.y;
       float ddz = pi.z - q.z;
       float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-      if (d2 <= P.rsq) f(s, q);
+      if (d2 <= P.rsq) f(s, q, r);
     }
   }
 }
 
-template <class F>
+template <int STRIDE, class F>
 __device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, float4 pi,
-                                              const float4* __restrict__ pos,
+                                              const float4* __restrict__ rec,
                                               const uint32_t* __restrict__ start,
                                               const uint32_t* __restrict__ nbr,
                                               const uint32_t* __restrict__ nn, F&& f) {
@@ -240,21 +272,24 @@ __device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, fl
     const uint32_t* l = nbr + i;
     const size_t stride = P.n;
     uint32_t t = 0;
-    // two neighbours per trip: both gathers in flight, bodies in list order
+    // two neighbours per trip: both records in flight, bodies in list order
     for (; t + 1u < c; t += 2u) {
       const uint32_t j0 = l[size_t(t) * stride];
       const uint32_t j1 = l[size_t(t + 1u) * stride];
-      const float4 q0 = pos[j0];
-      const float4 q1 = pos[j1];
-      f(j0, q0);
-      f(j1, q1);
+      const float4* r0 = rec + size_t(STRIDE) * j0;
+      const float4* r1 = rec + size_t(STRIDE) * j1;
+      const float4 q0 = r0[0];
+      const float4 q1 = r1[0];
+      f(j0, q0, r0);
+      f(j1, q1, r1);
     }
     if (t < c) {
       const uint32_t j0 = l[size_t(t) * stride];
-      f(j0, pos[j0]);
+      const float4* r0 = rec + size_t(STRIDE) * j0;
+      f(j0, r0[0], r0);
     }
   } else {
-    walk27r(P, i, pi, pos, start, f);
+    walk27r<STRIDE>(P, i, pi, rec, start, f);
   }
 }
 
@@ -269,18 +304,27 @@ __global__ UBP_LB void k_neigh(DevParams P, const float4* __restrict__ pos,
   pos_to_cell(P, pi.x, pi.y, pi.z, cx, cy, cz);
   Nbhd N;
   nbhd(P, cx, cy, cz, N);
+  float gx2[3], gy2[3], gz2[3];
+  slab_gaps(pi.x, P.dmin[0], P.cell, cx, P.dims[0], gx2);
+  slab_gaps(pi.y, P.dmin[1], P.cell, cy, P.dims[1], gy2);
+  slab_gaps(pi.z, P.dmin[2], P.cell, cz, P.dims[2], gz2);
+  const float skip2 = P.rsq * 1.0001f;
   uint32_t c = 0, ovf = 0;
   uint32_t* m = nmask + i;
 #pragma unroll 1
   for (int k = 0; k < 27; ++k) {
     const int dz = k / 9, dy = (k / 3) % 3, dx = k % 3;
     const uint32_t sx = sel3(N.sx, dx), sy = sel3(N.sy, dy), sz = sel3(N.sz, dz);
+    const float gap2 = (dx == 0 ? gx2[0] : (dx == 1 ? gx2[1] : gx2[2])) +
+                       (dy == 0 ? gy2[0] : (dy == 1 ? gy2[1] : gy2[2])) +
+                       (dz == 0 ? gz2[0] : (dz == 1 ? gz2[1] : gz2[2]));
     uint32_t mask = 0u;
-    if (sx != 0xffffffffu && sy != 0xffffffffu && sz != 0xffffffffu) {
+    if (sx != 0xffffffffu && sy != 0xffffffffu && sz != 0xffffffffu && !(gap2 > skip2)) {
       const uint32_t code = sx | sy | sz;
       const uint32_t b = start[code], e = start[code + 1u];
       if (e - b > 32u) ovf = UBP_NN_OVF;
-      for (uint32_t s = b; s < e; ++s) {
+      uint32_t bit = 1u;
+      for (uint32_t s = b; s < e; ++s, bit <<= 1) {
         if (s == i) continue;
         float4 q = pos[s];
         float ddx = pi.x - q.x;
@@ -289,7 +333,7 @@ __global__ UBP_LB void k_neigh(DevParams P, const float4* __restrict__ pos,
         float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
         if (d2 <= P.rsq) {
           ++c;
-          if (s - b < 32u) mask |= 1u << (s - b);
+          mask |= bit;  // bit is 0 beyond the 32nd slot of the cell
         }
       }
     }
@@ -359,22 +403,22 @@ __global__ __launch_bounds__(64) void k_expand(DevParams P, const float4* __rest
 // Density summation + delta-SPH (old density) + SDF wall volume + Tait EOS.
 // sph.rs:112-160 (density), sph.rs:169-216 (diffusion), lib.rs:232-237,
 // eos.rs:39-47.
-__global__ UBP_LB void k_density(DevParams P, const float4* __restrict__ pos,
-                                 const float2* __restrict__ rho_g, const float4* __restrict__ wall,
+__global__ UBP_LB void k_density(DevParams P, const float4* __restrict__ dr,
+                                 const float4* __restrict__ vel, const float4* __restrict__ wall,
                                  const uint8_t* __restrict__ ft, const uint32_t* __restrict__ start,
                                  const uint32_t* __restrict__ nbr, const uint32_t* __restrict__ nn,
                                  const float* __restrict__ dtp, float* __restrict__ rho_n,
-                                 float* __restrict__ prs, float* __restrict__ prho2) {
+                                 float* __restrict__ prs, float4* __restrict__ fr) {
   uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   uint32_t i = blk * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   const float dt = dtp[0];
-  float4 pi = pos[i];
-  const float rho_i = rho_g[i].x;
+  const float4 pi = dr[2u * i];
+  const float rho_i = dr[2u * i + 1u].x;
   Kahan rho, dif;
   rho.init(pi.w * P.norm_h3);  // m_i W(0, h)
   dif.init(0.0f);
-  for_neighbors(P, i, pi, pos, start, nbr, nn, [&](uint32_t j, float4 q) {
+  for_neighbors<2>(P, i, pi, dr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
     float dx = pi.x - q.x;
     float dy = pi.y - q.y;
     float dz = pi.z - q.z;
@@ -383,9 +427,9 @@ __global__ UBP_LB void k_density(DevParams P, const float4* __restrict__ pos,
     float qq = div_h(r, P);
     float gx, gy, gz;
     wgrad_q(dx, dy, dz, r, qq, P, gx, gy, gz);
-    const float2 rj = rho_g[j];
-    const float rho_j = rj.x;
-    const float v_j = rj.y;
+    const float4 rv = rj[1];
+    const float rho_j = rv.x;
+    const float v_j = rv.y;
     float r_dot_grad = dx * gx + dy * gy + dz * gz;
     float density_diff = rho_i - rho_j;
     dif.add(v_j * density_diff * 2.0f * r_dot_grad / (dist_sq + P.eta_sq));
@@ -416,29 +460,32 @@ __global__ UBP_LB void k_density(DevParams P, const float4* __restrict__ pos,
   float p = tait(rn, f, P);
   rho_n[i] = rn;
   prs[i] = p;
-  prho2[i] = p / (rn * rn);
+  // force record: (x y z m)(vx vy vz rho)(P/rho^2 P . .)
+  const float4 vi = vel[i];
+  fr[4u * i] = pi;
+  fr[4u * i + 1u] = make_float4(vi.x, vi.y, vi.z, rn);
+  fr[4u * i + 2u] = make_float4(p / (rn * rn), p, 0.0f, 0.0f);
 }
 
 // Pressure (sph.rs:322-417) + Monaghan viscosity (sph.rs:432-492) + gravity
 // (sph.rs:502-508) + SDF wall pressure and repulsion.
-__global__ UBP_LB void k_forces(DevParams P, const float4* __restrict__ pos,
-                                const float4* __restrict__ vel, const float* __restrict__ rho_n,
-                                const float* __restrict__ prs, const float* __restrict__ prho2,
+__global__ UBP_LB void k_forces(DevParams P, const float4* __restrict__ fr,
                                 const float4* __restrict__ wall, const uint8_t* __restrict__ ft,
                                 const uint32_t* __restrict__ start, const uint32_t* __restrict__ nbr,
                                 const uint32_t* __restrict__ nn, float4* __restrict__ acc) {
   uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   uint32_t i = blk * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
-  float4 pi = pos[i];
-  float4 vi = vel[i];
-  const float rho_i = rho_n[i];
-  const float pi_over = prho2[i];
+  const float4 pi = fr[4u * i];
+  const float4 vi = fr[4u * i + 1u];
+  const float rho_i = vi.w;
+  const float4 ei = fr[4u * i + 2u];
+  const float pi_over = ei.x;
   const float nmi = -pi.w;
   Kahan fx, fy, fz, vx, vy, vz;
   fx.init(0.0f); fy.init(0.0f); fz.init(0.0f);
   vx.init(0.0f); vy.init(0.0f); vz.init(0.0f);
-  for_neighbors(P, i, pi, pos, start, nbr, nn, [&](uint32_t j, float4 q) {
+  for_neighbors<4>(P, i, pi, fr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
     float dx = pi.x - q.x;
     float dy = pi.y - q.y;
     float dz = pi.z - q.z;
@@ -446,9 +493,10 @@ __global__ UBP_LB void k_forces(DevParams P, const float4* __restrict__ pos,
     float qq = div_h(r, P);
     float gx, gy, gz;
     wgrad_q(dx, dy, dz, r, qq, P, gx, gy, gz);
-    float factor = nmi * q.w * (pi_over + prho2[j]);
+    const float4 vj = rj[1];
+    const float4 ej = rj[2];
+    float factor = nmi * q.w * (pi_over + ej.x);
     fx.add(factor * gx); fy.add(factor * gy); fz.add(factor * gz);
-    float4 vj = vel[j];
     float dvx = vi.x - vj.x;
     float dvy = vi.y - vj.y;
     float dvz = vi.z - vj.z;
@@ -456,7 +504,7 @@ __global__ UBP_LB void k_forces(DevParams P, const float4* __restrict__ pos,
     const bool app = vr < 0.0f;  // approaching pairs only (sph.rs:467-468)
     float r_sq = r * r;
     float mu = P.h * vr / (r_sq + P.eta_sq);
-    float rho_avg = 0.5f * (rho_i + rho_n[j]);
+    float rho_avg = 0.5f * (rho_i + vj.w);
     float pij = (P.nac * mu + P.beta * mu * mu) / rho_avg;
     float vf = -q.w * pij;
     vx.add_if(app, vf * gx); vy.add_if(app, vf * gy); vz.add_if(app, vf * gz);
@@ -469,7 +517,7 @@ __global__ UBP_LB void k_forces(DevParams P, const float4* __restrict__ pos,
     uint32_t code = morton(P, cx, cy, cz);
     bk0 = P.bl_start[code];
     bk1 = P.bl_start[code + 1u];
-    const float pr = prs[i];
+    const float pr = ei.y;
     const float pic = (pr > 0.0f) ? pr : 0.0f;
     const float pic_over = pic / (rho_i * rho_i);
     const float brho = P.rho0[ft[i]];
@@ -512,7 +560,7 @@ __global__ UBP_LB void k_forces(DevParams P, const float4* __restrict__ pos,
   if (w.w != __builtin_inff()) {
     float s = w.w / P.h;
     if (s < 2.0f) {
-      float pr = prs[i];
+      float pr = ei.y;
       float pic = (pr > 0.0f) ? pr : 0.0f;
       int f = ft[i];
       float coef = 2.0f * (pic / (rho_i * rho_i)) * P.rho0[f] * wall_gamma(s) / P.h;
@@ -613,9 +661,11 @@ __global__ UBP_LB void k_adami(DevParams P, const float4* __restrict__ pos,
   bprs_out[b] = pb;
 }
 
-// v += a dt/2 (lib.rs:301-305)
+// v += a dt/2 (lib.rs:301-305); also writes the XSPH record
+// (x y z m)(vx vy vz rho) with the kicked velocity.
 __global__ UBP_LB void k_kick(DevParams P, float4* __restrict__ vel, const float4* __restrict__ acc,
-                              const float* __restrict__ dtp) {
+                              const float4* __restrict__ pos, const float* __restrict__ rho_n,
+                              const float* __restrict__ dtp, float4* __restrict__ xr) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
   const float half_dt = 0.5f * dtp[0];
@@ -625,6 +675,8 @@ __global__ UBP_LB void k_kick(DevParams P, float4* __restrict__ vel, const float
   v.y += a.y * half_dt;
   v.z += a.z * half_dt;
   vel[i] = v;
+  xr[2u * i] = pos[i];
+  xr[2u * i + 1u] = make_float4(v.x, v.y, v.z, rho_n[i]);
 }
 
 // Second half-kick (lib.rs:330-334) fused with the CFL reduction of
@@ -716,27 +768,26 @@ __global__ UBP_LB void k_cfl_finalize(DevParams P, const uint32_t* __restrict__ 
 
 // XSPH correction (sph.rs:236-280) on the post-kick velocities, over the
 // neighbour masks of the last density pass (same positions).
-__global__ UBP_LB void k_xsph(DevParams P, const float4* __restrict__ pos,
-                              const float4* __restrict__ vel, const float* __restrict__ rho_n,
+__global__ UBP_LB void k_xsph(DevParams P, const float4* __restrict__ xr,
                               const uint32_t* __restrict__ start, const uint32_t* __restrict__ nbr,
                               const uint32_t* __restrict__ nn, float4* __restrict__ dv) {
   uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   uint32_t i = blk * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
-  float4 pi = pos[i];
-  float4 vi = vel[i];
-  const float rho_i = rho_n[i];
+  const float4 pi = xr[2u * i];
+  const float4 vi = xr[2u * i + 1u];
+  const float rho_i = vi.w;
   float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-  for_neighbors(P, i, pi, pos, start, nbr, nn, [&](uint32_t j, float4 q) {
+  for_neighbors<2>(P, i, pi, xr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
     float dx = pi.x - q.x;
     float dy = pi.y - q.y;
     float dz = pi.z - q.z;
     float r = sqrtf(dx * dx + dy * dy + dz * dz);
     float w = wendland_q(div_h(r, P), P);
-    float rho_avg = 0.5f * (rho_i + rho_n[j]);
+    const float4 vj = rj[1];
+    float rho_avg = 0.5f * (rho_i + vj.w);
     float den = (rho_avg >= 1.0f) ? rho_avg : 1.0f;
     float factor = q.w / den * w;
-    float4 vj = vel[j];
     cx += factor * (vj.x - vi.x);
     cy += factor * (vj.y - vi.y);
     cz += factor * (vj.z - vi.z);
