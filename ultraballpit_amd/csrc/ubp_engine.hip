@@ -419,7 +419,7 @@ struct Engine {
       block = bs;
     }
     if (K && K != P.K) {
-      if (K < 1 || K > UBP_EXPAND_K) throw Error(UBP_EINVAL, "neighbour capacity must be in [1, 128]");
+      if (K < 1 || K > 1024) throw Error(UBP_EINVAL, "neighbour capacity must be in [1, 1024]");
       HIPCHK(hipStreamSynchronize(st));
       P.K = K;
       alloc_nbr();

@@ -261,7 +261,7 @@ __device__ __forceinline__ void walk27r(const DevParams& P, uint32_t i, float4 p
   }
 }
 
-template <int STRIDE, class F>
+template <int STRIDE, int UNROLL = 2, class F>
 __device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, float4 pi,
                                               const float4* __restrict__ rec,
                                               const uint32_t* __restrict__ start,
@@ -272,18 +272,20 @@ __device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, fl
     const uint32_t* l = nbr + i;
     const size_t stride = P.n;
     uint32_t t = 0;
-    // two neighbours per trip: both records in flight, bodies in list order
-    for (; t + 1u < c; t += 2u) {
-      const uint32_t j0 = l[size_t(t) * stride];
-      const uint32_t j1 = l[size_t(t + 1u) * stride];
-      const float4* r0 = rec + size_t(STRIDE) * j0;
-      const float4* r1 = rec + size_t(STRIDE) * j1;
-      const float4 q0 = r0[0];
-      const float4 q1 = r1[0];
-      f(j0, q0, r0);
-      f(j1, q1, r1);
+    // UNROLL neighbours per trip: their records in flight together, bodies
+    // in list order
+    for (; t + UNROLL <= c; t += UNROLL) {
+      uint32_t j[UNROLL];
+      const float4* r[UNROLL];
+      float4 q[UNROLL];
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) j[u] = l[size_t(t + u) * stride];
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) { r[u] = rec + size_t(STRIDE) * j[u]; q[u] = r[u][0]; }
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) f(j[u], q[u], r[u]);
     }
-    if (t < c) {
+    for (; t < c; ++t) {
       const uint32_t j0 = l[size_t(t) * stride];
       const float4* r0 = rec + size_t(STRIDE) * j0;
       f(j0, r0[0], r0);
@@ -304,22 +306,14 @@ __global__ UBP_LB void k_neigh(DevParams P, const float4* __restrict__ pos,
   pos_to_cell(P, pi.x, pi.y, pi.z, cx, cy, cz);
   Nbhd N;
   nbhd(P, cx, cy, cz, N);
-  float gx2[3], gy2[3], gz2[3];
-  slab_gaps(pi.x, P.dmin[0], P.cell, cx, P.dims[0], gx2);
-  slab_gaps(pi.y, P.dmin[1], P.cell, cy, P.dims[1], gy2);
-  slab_gaps(pi.z, P.dmin[2], P.cell, cz, P.dims[2], gz2);
-  const float skip2 = P.rsq * 1.0001f;
   uint32_t c = 0, ovf = 0;
   uint32_t* m = nmask + i;
 #pragma unroll 1
   for (int k = 0; k < 27; ++k) {
     const int dz = k / 9, dy = (k / 3) % 3, dx = k % 3;
     const uint32_t sx = sel3(N.sx, dx), sy = sel3(N.sy, dy), sz = sel3(N.sz, dz);
-    const float gap2 = (dx == 0 ? gx2[0] : (dx == 1 ? gx2[1] : gx2[2])) +
-                       (dy == 0 ? gy2[0] : (dy == 1 ? gy2[1] : gy2[2])) +
-                       (dz == 0 ? gz2[0] : (dz == 1 ? gz2[1] : gz2[2]));
     uint32_t mask = 0u;
-    if (sx != 0xffffffffu && sy != 0xffffffffu && sz != 0xffffffffu && !(gap2 > skip2)) {
+    if (sx != 0xffffffffu && sy != 0xffffffffu && sz != 0xffffffffu) {
       const uint32_t code = sx | sy | sz;
       const uint32_t b = start[code], e = start[code + 1u];
       if (e - b > 32u) ovf = UBP_NN_OVF;
@@ -342,29 +336,34 @@ __global__ UBP_LB void k_neigh(DevParams P, const float4* __restrict__ pos,
   nn[i] = c | ovf;
 }
 
-// Masks -> flat lists.  One wavefront per 64-thread workgroup; each lane
-// decodes its masks in order into an LDS column, then the tile is stored row
-// by row (t = 0, 1, ...), 64 consecutive slots per row.
-#define UBP_EXPAND_K 128
+// Masks -> flat lists.  One wavefront per 64-thread workgroup.  The list
+// rows are produced in rounds of 32: in round r every lane decodes its masks
+// (held in registers) into entries [32 r, 32 r + 32) of its list, parks them
+// in a 32 x 64 LDS tile, and the wavefront then stores the 32 rows, 64
+// consecutive slots per row.
+#define UBP_EXPAND_CHUNK 32
 __global__ __launch_bounds__(64) void k_expand(DevParams P, const float4* __restrict__ pos,
                                                const uint32_t* __restrict__ start,
                                                const uint32_t* __restrict__ nmask,
                                                const uint32_t* __restrict__ nn,
                                                uint32_t* __restrict__ nbr) {
-  __shared__ uint32_t tile[UBP_EXPAND_K][64];
+  __shared__ uint32_t tile[UBP_EXPAND_CHUNK][64];
   const uint32_t lane = threadIdx.x;
   const uint32_t i = blockIdx.x * 64u + lane;
-  uint32_t c = 0;
+  uint32_t mk[27], bk[27];
+  uint32_t cnt = 0;
+#pragma unroll
+  for (int k = 0; k < 27; ++k) { mk[k] = 0u; bk[k] = 0u; }
   if (i < P.n) {
     const uint32_t nv = nn[i];
     if (nv <= P.K) {
+      cnt = nv;
       const float4 pi = pos[i];
       uint32_t cx, cy, cz;
       pos_to_cell(P, pi.x, pi.y, pi.z, cx, cy, cz);
       Nbhd N;
       nbhd(P, cx, cy, cz, N);
       const uint32_t* m = nmask + i;
-      uint32_t mk[27], bk[27];
 #pragma unroll
       for (int k = 0; k < 27; ++k) mk[k] = m[size_t(k) * P.n];  // 27 loads in flight
 #pragma unroll
@@ -373,30 +372,36 @@ __global__ __launch_bounds__(64) void k_expand(DevParams P, const float4* __rest
         const uint32_t code = N.sx[dx] | N.sy[dy] | N.sz[dz];
         bk[k] = (mk[k] != 0u) ? start[code] : 0u;  // masks are 0 off the grid
       }
-#pragma unroll
-      for (int k = 0; k < 27; ++k) {
-        uint32_t mask = mk[k];
-        const uint32_t b = bk[k];
-        while (mask) {
-          const uint32_t bit = uint32_t(__builtin_ctz(mask));
-          mask &= mask - 1u;
-          tile[c][lane] = b + bit;
-          ++c;
-        }
-      }
     }
   }
-  // longest list in the wavefront
-  uint32_t tmax = c;
+  uint32_t tmax = cnt;  // longest list in the wavefront
   for (int o = 32; o > 0; o >>= 1) {
     uint32_t t = __shfl_xor(tmax, o, 64);
     tmax = tmax > t ? tmax : t;
   }
-  __syncthreads();
-  if (i < P.n) {
-    uint32_t* l = nbr + i;
-    for (uint32_t t = 0; t < tmax; ++t)
-      if (t < c) l[size_t(t) * P.n] = tile[t][lane];
+  uint32_t* l = nbr + i;
+  for (uint32_t r0 = 0; r0 < tmax; r0 += UBP_EXPAND_CHUNK) {
+    // entries [r0, r0 + 32) of this lane's list
+    uint32_t skip = r0, e = 0;
+#pragma unroll
+    for (int k = 0; k < 27; ++k) {
+      uint32_t mask = mk[k];
+      const uint32_t pc = uint32_t(__popc(mask));
+      if (skip >= pc) { skip -= pc; continue; }
+      for (; skip > 0u; --skip) mask &= mask - 1u;
+      while (mask && e < UBP_EXPAND_CHUNK) {
+        const uint32_t bit = uint32_t(__builtin_ctz(mask));
+        mask &= mask - 1u;
+        tile[e][lane] = bk[k] + bit;
+        ++e;
+      }
+    }
+    __syncthreads();
+    const uint32_t rows = (tmax - r0 < UBP_EXPAND_CHUNK) ? tmax - r0 : UBP_EXPAND_CHUNK;
+    if (i < P.n)
+      for (uint32_t t = 0; t < rows; ++t)
+        if (r0 + t < cnt) l[size_t(r0 + t) * P.n] = tile[t][lane];
+    __syncthreads();
   }
 }
 
@@ -778,7 +783,7 @@ __global__ UBP_LB void k_xsph(DevParams P, const float4* __restrict__ xr,
   const float4 vi = xr[2u * i + 1u];
   const float rho_i = vi.w;
   float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-  for_neighbors<2>(P, i, pi, xr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
+  for_neighbors<2, 4>(P, i, pi, xr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
     float dx = pi.x - q.x;
     float dy = pi.y - q.y;
     float dz = pi.z - q.z;
