@@ -35,12 +35,13 @@ HBM_PEAK_GBS = 8000.0
 # the neighbour masks (a design choice, not algorithmic) are not counted.
 ALG_BYTES = {
     "k_kick": 36,              # R vel 12, acc 12; W vel 12
-    "k_xsph+k_drift": 108,     # XSPH: R pos, vel, rho, m, W dv (44); drift: R pos, vel, dv,
-                               # W pos, vel (60); W cell key (4)
+    "k_xsph": 44,              # R pos 12, vel 12, rho 4, m 4; W dv 12
+    "k_drift": 64,             # R pos, vel, dv 36; W pos, vel 24; W cell key 4
     "k_scan": 0,
     "k_scatter": 8,            # R key, W permutation
     "k_rank_gather": 64,       # SoA gather, R + W pos, vel, rho, m
-    "k_neigh+k_expand": 12,    # R pos (the masks and lists are not algorithmic)
+    "k_neigh": 12,             # R pos (the masks and lists are not algorithmic)
+    "k_expand": 0,
     "k_density": 28,           # R pos 12, rho_old 4, m 4; W rho 4, P 4
     "k_forces": 48,            # R pos 12, vel 12, rho 4, P 4, m 4; W acc 12
     "k_kick2_cfl": 36,         # R vel 12, acc 12; W vel 12
@@ -55,13 +56,24 @@ def log(*a):
 
 
 def traffic_from_profiles(kernel: str):
-    """HBM bytes per dispatch of `kernel` from the committed rocprof PMC
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor)."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_dam_break_1m_pmc.txt")
+    """HBM bytes per dispatch of `kernel` from the committed rocprof summary
+    of this workload (profiles/step_kernels_dam_break_1m.md: FETCH_SIZE and
+    WRITE_SIZE per dispatch, already scaled by the copy-kernel factor, MiB)."""
+    path = os.path.join(ROOT, "profiles", "step_kernels_dam_break_1m.md")
     if not os.path.exists(path):
         return None
-    m = re.search(r"^traffic_bytes_per_dispatch:\s*([0-9.eE+]+)", open(path).read(), re.M)
-    return float(m.group(1)) if m else None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) < 6:
+            continue
+        name = re.sub(r"^(void )?ubp::", "", cells[0])
+        name = re.sub(r"<.*>$", "", name)
+        if name == kernel:
+            try:
+                return (float(cells[4]) + float(cells[5])) * 2.0 ** 20
+            except ValueError:
+                return None
+    return None
 
 
 def cpu_baseline(scene, budget_s: float):
@@ -87,6 +99,36 @@ def cpu_baseline(scene, budget_s: float):
                       f"C++ oracle (engine arithmetic), OpenMP {threads} threads"}
 
 
+def sweep_bench(steps: int, warmup: int, trial: int):
+    """Batched parameter sweep: 256 dam-break scenes (log-uniform 2k-200k
+    particles, fill height and alpha drawn per scene) in one engine, per-scene
+    dt on the device, graph-replayed steps, launch shapes from the autotuner."""
+    t0 = time.perf_counter()
+    sw = scenes.sweep_scenes()
+    total = int(sum(len(s["x"]) for s in sw))
+    b = ubp.HipBatch(sw)
+    t1 = time.perf_counter()
+    tuning = b.autotune(trial_steps=trial) if trial > 0 else b.get_tuning()
+    t2 = time.perf_counter()
+    b.step_n(warmup, use_graph=True)
+    b.sync()
+    t3 = time.perf_counter()
+    b.step_n(steps, use_graph=True)
+    b.sync()
+    t4 = time.perf_counter()
+    dts = b.dts()
+    s = b.particles()
+    finite = bool(all(np.isfinite(s[k]).all() for k in ("x", "y", "z", "vx", "vy", "vz", "density")))
+    b.close()
+    return {"workload": "256 dam-break scenes, N_s log-uniform in [2e3, 2e5], fill U[0.3, 0.7] of the "
+                        "1.0x0.25x0.6 m tank, alpha U[0.005, 0.05], per-scene device dt, one engine",
+            "scenes": len(sw), "particles": total, "steps": steps, "warmup": warmup,
+            "value": total * steps / (t4 - t3), "unit": "particle-steps/s",
+            "ms_per_step": (t4 - t3) * 1e3 / steps, "tuning": tuning,
+            "create_s": t1 - t0, "autotune_s": t2 - t1,
+            "dt_min": float(dts.min()), "dt_max": float(dts.max()), "state_finite": finite}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,11 +138,22 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--sweep-steps", type=int, default=50,
+                    help="steps of the 256-scene batched sweep (0 skips it)")
+    ap.add_argument("--sweep-only", action="store_true")
+    ap.add_argument("--autotune-steps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(see DESIGN.md for how the dam break would shard)", file=sys.stderr)
         sys.exit(2)
+
+    if args.sweep_only:
+        log(f"[bench] batched sweep, {args.sweep_steps} steps")
+        out = sweep_bench(args.sweep_steps, args.warmup, args.autotune_steps)
+        out["metric"] = "particle-steps/s, 256-scene batched sweep"
+        print(json.dumps(out), flush=True)
+        return
 
     log(f"[bench] building dam-break scene ({args.particles} particles)")
     sc = scenes.dam_break(n_keep=args.particles)
@@ -128,7 +181,8 @@ def main():
     t2 = time.perf_counter()
     kms = eng.step_n_profiled(args.steps)
     t3 = time.perf_counter()
-    per_kernel_us = {k: v * 1e3 / args.steps for k, v in kms.items()}
+    timed = max(1, eng.timed_steps)
+    per_kernel_us = {k: v * 1e3 / timed for k, v in kms.items()}
     dom = max(per_kernel_us, key=per_kernel_us.get)
     avg_s = per_kernel_us[dom] * 1e-6
     alg = ALG_BYTES.get(dom, 0) * n
@@ -163,10 +217,13 @@ def main():
         "state_finite": finite,
         "cpu_baseline": None,
     }
+    eng.close()
+    if args.sweep_steps > 0:
+        log(f"[bench] batched sweep, {args.sweep_steps} steps")
+        out["sweep"] = sweep_bench(args.sweep_steps, min(args.warmup, 10), args.autotune_steps)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) on the host cores")
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_budget)
-    eng.close()
     print(json.dumps(out), flush=True)
 
 

@@ -139,10 +139,12 @@ int ubp_step_profiled(ubp_engine *e, float dt, ubp_step_profile *out);
  * benches/gpu_throughput.rs:136-161).  HIP events are recorded on the
  * handle's stream around every launch; kernel_ms[k] receives the summed
  * milliseconds of kernel k over the n steps, in the order:
- *   0 kick, 1 xsph + drift, 2 scan (3 launches), 3 scatter, 4 rank_gather,
- *   5 neigh + expand, 6 density (+ adami), 7 forces, 8 kick2_cfl + cfl_finalize.
+ *   0 kick, 1 xsph, 2 drift, 3 scan (3 launches), 4 scatter, 5 rank_gather,
+ *   6 neigh, 7 expand, 8 density (+ adami), 9 forces, 10 kick2_cfl + cfl_finalize.
+ * Returns the number of steps timed (>= 0): the first step of an engine that
+ * has not stepped yet carries the bootstrap force pass and runs untimed.
  * Synchronous. */
-#define UBP_NKERN 9
+#define UBP_NKERN 11
 int ubp_step_n_profiled(ubp_engine *e, int nsteps, double *kernel_ms);
 
 /* Measurement utility, no reference counterpart: `reps` launches of a plain
@@ -172,8 +174,61 @@ int ubp_restore_checkpoint(ubp_engine *e);
  * NeighborGrid::{cell_indices, sorted_indices} (neighbor.rs:11-23). */
 int ubp_debug_bins(ubp_engine *e, uint32_t *keys, uint32_t *perm, uint32_t *nbr_counts);
 
-/* Grid geometry of the handle: dims[3], Morton bits[3]. */
+/* Grid geometry of the handle: dims[3], Morton bits[3] (scene 0 of a batch). */
 int ubp_grid_info(const ubp_engine *e, uint32_t dims[3], uint32_t bits[3]);
+
+/* ---- Many scenes per handle (no reference counterpart: the reference runs
+ * one scene per kernel object; a parameter sweep such as
+ * orchestrator/src/distributed.rs:172-188 creates one kernel per case).
+ * A batch handle steps every scene by one step per step, each scene with its
+ * own dt (ubp_step_n) or all with the caller's dt (ubp_step).  Results per
+ * scene are bit-identical to a one-scene handle of the same scene.  The
+ * trait calls above work on a batch handle over the concatenation of the
+ * scenes' particles (scene s occupies [offset_s, offset_s + n_s) in input
+ * order); boundary particles are not supported in a batch. */
+typedef struct {
+  const ubp_particles_in *particles;
+  float h;
+  float gravity[3];
+  float speed_of_sound;
+  float cfl_number;
+  float domain_min[3];
+  float domain_max[3];
+  float alpha, beta;          /* Monaghan coefficients (sph.rs:440-441) */
+  const float *sdf;           /* optional SDF walls (NULL: none), layout as ubp_set_sdf */
+  float sdf_origin[3];
+  float sdf_cell;
+  uint32_t sdf_dims[3];
+} ubp_scene;
+
+ubp_engine *ubp_create_batch(const ubp_scene *scenes, uint32_t nscenes);
+/* Number of scenes (1 for a handle made by ubp_create). */
+uint32_t ubp_scene_count(const ubp_engine *e);
+/* Input-order particle range of scene s. */
+int ubp_scene_range(const ubp_engine *e, uint32_t s, size_t *offset, size_t *count);
+/* dt of every scene for the next ubp_step_n step, dts[nscenes]; synchronous. */
+int ubp_get_scene_dts(ubp_engine *e, float *dts);
+/* error_metrics of scene s (lib.rs:341-377). */
+int ubp_scene_error_metrics(ubp_engine *e, uint32_t s, ubp_error_metrics_t *out);
+
+/* ---- Launch-shape autotuner (successor of set_workgroup_size,
+ * backend/crates/kernel/src/gpu/mod.rs:711-767, and of the workgroup sweep
+ * bench, benches/workgroup_sweep.rs:33-82).  Passes: */
+#define UBP_NPASS 5 /* 0 elementwise (kick, drift, key, sort, gather, kick2), 1 neighbour
+                       masks, 2 density, 3 forces, 4 XSPH */
+typedef struct {
+  uint32_t block[UBP_NPASS]; /* workgroup size per pass: 64, 128 or 256 */
+  uint32_t nbr_capacity;     /* neighbour-list slots per particle */
+  double step_ms;            /* event-timed ms per step of the final trial (autotune only) */
+} ubp_tuning;
+/* Picks the list capacity from one step's neighbour counts, then the
+ * workgroup size of each pass from `trial_steps` event-timed device-dt steps
+ * per candidate.  The state is snapshotted and restored: the simulation does
+ * not advance, and every choice gives bit-identical results.  Synchronous. */
+int ubp_autotune(ubp_engine *e, int trial_steps, ubp_tuning *out);
+int ubp_get_tuning(const ubp_engine *e, ubp_tuning *out);
+/* Workgroup size per pass (block[UBP_NPASS]); nbr_capacity 0 keeps it. */
+int ubp_set_tuning_ex(ubp_engine *e, const ubp_tuning *t);
 
 #ifdef __cplusplus
 }

@@ -135,3 +135,33 @@ def test_sweep_scenes_counts():
     assert min(ns) >= 1000 and max(ns) <= 40000
     alphas = [float(s["alpha"]) for s in sw]
     assert all(0.005 <= a <= 0.05 for a in alphas)
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """ctypes mirrors of ubp_scene / ubp_tuning / ubp_particles_in have the
+    C layout of include/ubp.h (compiled with the host compiler)."""
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "ubp.h"\n'
+        "int main(void){\n"
+        ' printf("%zu %zu %zu %zu %zu\\n", sizeof(ubp_scene), offsetof(ubp_scene, alpha),'
+        " offsetof(ubp_scene, sdf), offsetof(ubp_scene, sdf_cell), offsetof(ubp_scene, sdf_dims));\n"
+        ' printf("%zu %zu %zu\\n", sizeof(ubp_tuning), offsetof(ubp_tuning, nbr_capacity),'
+        " offsetof(ubp_tuning, step_ms));\n"
+        ' printf("%zu %zu\\n", sizeof(ubp_particles_in), offsetof(ubp_particles_in, fluid_type));\n'
+        " return 0; }\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [list(map(int, l.split())) for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                             check=True).stdout.splitlines()]
+    S, T, P = ubp._Scene, ubp._Tuning, ubp._ParticlesIn
+    assert got[0] == [C.sizeof(S), S.alpha.offset, S.sdf.offset, S.sdf_cell.offset, S.sdf_dims.offset]
+    assert got[1] == [C.sizeof(T), T.nbr_capacity.offset, T.step_ms.offset]
+    assert got[2] == [C.sizeof(P), P.fluid_type.offset]
+
+
+@pytest.mark.skipif(ubp.gpu_available(), reason="CPU-only check")
+def test_batch_create_fails_cleanly_without_device():
+    sw = scenes.sweep_scenes(n_scenes=2, n_min=100, n_max=300)
+    with pytest.raises(ubp.GpuInitError, match="device"):
+        ubp.HipBatch(sw)

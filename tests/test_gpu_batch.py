@@ -1,0 +1,196 @@
+"""Many scenes per engine and the launch-shape autotuner.
+
+A batch engine concatenates independent scenes into one slot space with
+disjoint Morton code ranges; every pass runs once for all scenes and each
+scene has its own dt.  Its contract is per-scene equality: every scene of a
+batch must reproduce, bit for bit, a one-scene engine of the same scene
+(which the parity tests hold bit-exact against the oracle).  The autotuner
+must not advance the state, and any workgroup size it picks must give the
+same bits.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import pyoracle
+import ultraballpit_amd as ubp
+from ultraballpit_amd import scenes
+from parity_util import STATE, assert_bitwise, first_mismatch, oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not ubp.gpu_available():
+        pytest.fail("no gfx950 device / engine library not loadable")
+
+
+def small_sweep(k=6, n_min=400, n_max=5000, seed=3):
+    return scenes.sweep_scenes(n_scenes=k, n_min=n_min, n_max=n_max, seed=seed)
+
+
+def assert_scene_equal(got: dict, want: dict, where: str):
+    mm = first_mismatch(got, want)
+    assert mm is None, f"{where}: first mismatch {mm} (field, index, batch, single, count)"
+
+
+def test_batch_matches_single_scene_engines_device_dt():
+    sw = small_sweep()
+    say("case: batch of", len(sw), "dam-break scenes", [len(s["x"]) for s in sw], "30 device-dt steps, graph")
+    b = ubp.HipBatch(sw)
+    assert b.scene_count() == len(sw)
+    dts0 = b.dts()
+    b.step_n(30, use_graph=True)
+    dts = b.dts()
+    parts = b.scene_particles()
+    for s, sc in enumerate(sw):
+        e = ubp.HipKernel.from_scene(sc)
+        assert np.float32(e.dt()) == dts0[s], f"scene {s}: initial dt"
+        e.step_n(30, use_graph=True)
+        assert_scene_equal(parts[s], e.particles(), f"scene {s} (n={len(sc['x'])})")
+        assert np.float32(e.dt()) == dts[s], f"scene {s}: dt after 30 steps"
+        e.close()
+    assert len(set(dts.tolist())) > 1, "scenes should run with different dt"
+    b.close()
+
+
+def test_batch_scene_matches_oracle():
+    sw = small_sweep(k=3, n_min=300, n_max=1500, seed=8)
+    say("case: batch of 3 scenes vs the oracle per scene, 15 device-dt steps")
+    b = ubp.HipBatch(sw)
+    b.step_n(15, use_graph=False)
+    parts = b.scene_particles()
+    for s, sc in enumerate(sw):
+        o = oracle(sc)
+        o.step_auto(15)
+        assert_bitwise(parts[s], o, f"batch scene {s} vs oracle")
+    b.close()
+
+
+def test_batch_host_dt_and_checkpoint():
+    sw = small_sweep(k=4, seed=5)
+    say("case: batch host-dt steps dt=2e-4 and checkpoint round trip")
+    b = ubp.HipBatch(sw)
+    singles = [ubp.HipKernel.from_scene(sc) for sc in sw]
+    for _ in range(5):
+        b.step(2e-4)
+        for e in singles:
+            e.step(2e-4)
+    assert b.save_checkpoint()
+    ref = b.particles()
+    b.step_n(7, use_graph=False)
+    moved = b.particles()
+    assert first_mismatch(moved, ref) is not None
+    assert b.restore_checkpoint()
+    assert first_mismatch(b.particles(), ref) is None
+    b.step_n(7, use_graph=True)
+    again = b.scene_particles()
+    for s, e in enumerate(singles):
+        e.step_n(7, use_graph=True)
+        assert_scene_equal(again[s], e.particles(), f"scene {s} after restore + 7 steps")
+        e.close()
+    b.close()
+
+
+def test_batch_metrics_per_scene():
+    sw = small_sweep(k=3, seed=9)
+    say("case: per-scene error metrics of a batch")
+    b = ubp.HipBatch(sw)
+    b.step_n(10, use_graph=False)
+    for s, sc in enumerate(sw):
+        e = ubp.HipKernel.from_scene(sc)
+        e.step_n(10, use_graph=False)
+        mb, me = b.scene_error_metrics(s), e.error_metrics()
+        assert mb.max_density_variation == me.max_density_variation
+        # FP64 sums over different block partitions: equal to rounding
+        assert mb.energy_conservation == pytest.approx(me.energy_conservation, rel=1e-6, abs=1e-12)
+        assert mb.mass_conservation == pytest.approx(me.mass_conservation, abs=1e-12)
+        e.close()
+    b.close()
+
+
+def test_batch_ragged_sizes_and_single_particle_scene():
+    sw = small_sweep(k=2, seed=4)
+    one = scenes.dam_break_small(lat=(1, 1, 1))
+    sw.append(one)
+    say("case: batch with a one-particle scene, 12 steps")
+    b = ubp.HipBatch(sw)
+    b.step_n(12, use_graph=True)
+    parts = b.scene_particles()
+    for s, sc in enumerate(sw):
+        e = ubp.HipKernel.from_scene(sc)
+        e.step_n(12, use_graph=True)
+        assert_scene_equal(parts[s], e.particles(), f"scene {s} (n={len(sc['x'])})")
+        e.close()
+    b.close()
+
+
+def test_autotune_keeps_state_and_bits():
+    sc = scenes.dam_break_small(lat=(14, 14, 28))
+    say("case: autotune on dam_break_small; state unchanged, same bits afterwards")
+    a = ubp.HipKernel.from_scene(sc)
+    a.step_n(3, use_graph=False)
+    before = a.particles()
+    t = a.autotune(trial_steps=3)
+    assert set(t["block"].values()) <= {64, 128, 256}
+    assert 32 <= t["nbr_capacity"] <= 1024 and t["step_ms"] > 0
+    assert a.get_tuning()["block"] == t["block"]
+    assert first_mismatch(a.particles(), before) is None, "autotune advanced the state"
+    ref = ubp.HipKernel.from_scene(sc)
+    ref.step_n(3, use_graph=False)
+    a.step_n(20, use_graph=True)
+    ref.step_n(20, use_graph=True)
+    assert first_mismatch(a.particles(), ref.particles()) is None
+    a.close()
+    ref.close()
+
+
+def test_autotune_before_first_step():
+    sc = scenes.dam_break_small(lat=(10, 10, 20))
+    say("case: autotune before the first step, then 10 steps vs the oracle")
+    a = ubp.HipKernel.from_scene(sc)
+    a.autotune(trial_steps=3)
+    a.step_n(10, use_graph=False)
+    o = oracle(sc)
+    o.step_auto(10)
+    assert_bitwise(a.particles(), o, "after autotune + 10 steps")
+    a.close()
+
+
+@pytest.mark.parametrize("blocks", [(64, 64, 64, 64, 64), (128, 256, 64, 128, 256), (256, 64, 128, 256, 64)])
+def test_pass_blocks_same_bits(blocks):
+    sw = small_sweep(k=3, seed=12)
+    say("case: batch with per-pass blocks", blocks)
+    b = ubp.HipBatch(sw)
+    b.set_pass_blocks(blocks, nbr_capacity=96)
+    b.step_n(10, use_graph=True)
+    parts = b.scene_particles()
+    for s, sc in enumerate(sw):
+        e = ubp.HipKernel.from_scene(sc)
+        e.step_n(10, use_graph=True)
+        assert_scene_equal(parts[s], e.particles(), f"scene {s} with blocks {blocks}")
+        e.close()
+    b.close()
+
+
+def test_profiled_steps_same_bits_and_count():
+    sw = small_sweep(k=2, seed=21)
+    say("case: step_n_profiled on a fresh batch vs step_n")
+    a = ubp.HipBatch(sw)
+    b = ubp.HipBatch(sw)
+    kms = a.step_n_profiled(6)
+    assert a.timed_steps == 5
+    assert all(v >= 0.0 for v in kms.values()) and kms["k_forces"] > 0.0
+    b.step_n(6, use_graph=True)
+    assert first_mismatch(a.particles(), b.particles()) is None
+    kms = a.step_n_profiled(3)
+    assert a.timed_steps == 3
+    a.close()
+    b.close()

@@ -6,6 +6,10 @@ relative to the reference repository):
 * ``HipKernel``      -- the ``SimulationKernel`` trait (backend/crates/kernel/
                         src/lib.rs:58-78) plus the ``GpuKernel`` extras
                         (gpu/mod.rs:996-1278), over the C ABI in include/ubp.h;
+* ``HipBatch``       -- many independent scenes stepped together (one launch
+                        per pass for all scenes, per-scene dt), and the
+                        launch-shape autotuner (successor of
+                        ``set_workgroup_size``, gpu/mod.rs:711-767);
 * ``gpu_available``  -- gpu/mod.rs:171-182;
 * ``BackendType`` / ``create_kernel`` -- backend selection of
                         orchestrator/src/config.rs:9-22 and
@@ -74,6 +78,23 @@ class _Profile(C.Structure):
                                           "forces_us", "integrate_us", "readback_us", "total_us")]
 
 
+NPASS = 5
+#: passes of ubp_tuning.block
+PASSES = ("elementwise", "neighbour_masks", "density", "forces", "xsph")
+
+
+class _Tuning(C.Structure):
+    _fields_ = [("block", C.c_uint32 * NPASS), ("nbr_capacity", C.c_uint32), ("step_ms", C.c_double)]
+
+
+class _Scene(C.Structure):
+    _fields_ = [("particles", C.POINTER(_ParticlesIn)), ("h", C.c_float), ("gravity", C.c_float * 3),
+                ("speed_of_sound", C.c_float), ("cfl_number", C.c_float),
+                ("domain_min", C.c_float * 3), ("domain_max", C.c_float * 3),
+                ("alpha", C.c_float), ("beta", C.c_float), ("sdf", C.POINTER(C.c_float)),
+                ("sdf_origin", C.c_float * 3), ("sdf_cell", C.c_float), ("sdf_dims", C.c_uint32 * 3)]
+
+
 _lib = None
 
 #: every symbol include/ubp.h declares
@@ -82,10 +103,12 @@ EXPORTS = ("ubp_create", "ubp_destroy", "ubp_last_error", "ubp_gpu_available",
            "ubp_step_async", "ubp_sync", "ubp_step_n", "ubp_get_dt", "ubp_step_profiled",
            "ubp_read_particles", "ubp_particle_count", "ubp_error_metrics",
            "ubp_save_checkpoint", "ubp_restore_checkpoint", "ubp_debug_bins", "ubp_grid_info",
-           "ubp_step_n_profiled", "ubp_calibration_copy")
+           "ubp_step_n_profiled", "ubp_calibration_copy", "ubp_create_batch", "ubp_scene_count",
+           "ubp_scene_range", "ubp_get_scene_dts", "ubp_scene_error_metrics", "ubp_autotune",
+           "ubp_get_tuning", "ubp_set_tuning_ex")
 
 #: kernel order of ubp_step_n_profiled
-KERNELS = ("k_kick", "k_xsph+k_drift", "k_scan", "k_scatter", "k_rank_gather", "k_neigh+k_expand",
+KERNELS = ("k_kick", "k_xsph", "k_drift", "k_scan", "k_scatter", "k_rank_gather", "k_neigh", "k_expand",
            "k_density", "k_forces", "k_kick2_cfl")
 
 
@@ -131,6 +154,16 @@ def lib():
     L.ubp_grid_info.argtypes = [C.c_void_p, u32p, u32p]
     L.ubp_step_n_profiled.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
     L.ubp_calibration_copy.argtypes = [C.c_size_t, C.c_int, C.POINTER(C.c_double)]
+    L.ubp_create_batch.restype = C.c_void_p
+    L.ubp_create_batch.argtypes = [C.POINTER(_Scene), C.c_uint32]
+    L.ubp_scene_count.restype = C.c_uint32
+    L.ubp_scene_count.argtypes = [C.c_void_p]
+    L.ubp_scene_range.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+    L.ubp_get_scene_dts.argtypes = [C.c_void_p, fp]
+    L.ubp_scene_error_metrics.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(_Metrics)]
+    L.ubp_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(_Tuning)]
+    L.ubp_get_tuning.argtypes = [C.c_void_p, C.POINTER(_Tuning)]
+    L.ubp_set_tuning_ex.argtypes = [C.c_void_p, C.POINTER(_Tuning)]
     for name in EXPORTS:
         getattr(L, name)
     _lib = L
@@ -167,6 +200,37 @@ def _u32p(a):
     return a.ctypes.data_as(C.POINTER(C.c_uint32))
 
 
+def _particles_in(particles: dict, keep: list) -> _ParticlesIn:
+    """ParticleArrays-shaped dict -> ubp_particles_in (arrays kept alive in `keep`)."""
+    n = int(len(particles["x"]))
+
+    def arr(k):
+        v = particles.get(k)
+        if v is None:
+            return None
+        a = _f32(v)
+        if a.shape != (n,):
+            raise GpuInitError(f"particle array {k} has shape {a.shape}, expected ({n},)")
+        keep.append(a)
+        return _fp(a)
+
+    pin = _ParticlesIn()
+    pin.n = n
+    for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "density", "mass", "temperature"):
+        setattr(pin, k, arr(k))
+    ft = particles.get("fluid_type")
+    if ft is not None:
+        fta = np.ascontiguousarray(ft, dtype=np.uint8)
+        keep.append(fta)
+        pin.fluid_type = fta.ctypes.data_as(C.POINTER(C.c_uint8))
+    return pin
+
+
+def _tuning_dict(t: _Tuning) -> dict:
+    return {"block": dict(zip(PASSES, [int(b) for b in t.block])), "nbr_capacity": int(t.nbr_capacity),
+            "step_ms": float(t.step_ms)}
+
+
 class HipKernel:
     """``SimulationKernel`` on the MI355X engine.
 
@@ -183,26 +247,7 @@ class HipKernel:
         n = int(len(particles["x"]))
         self._n = n
         keep = []
-
-        def arr(k):
-            v = particles.get(k)
-            if v is None:
-                return None
-            a = _f32(v)
-            if a.shape != (n,):
-                raise GpuInitError(f"particle array {k} has shape {a.shape}, expected ({n},)")
-            keep.append(a)
-            return _fp(a)
-
-        pin = _ParticlesIn()
-        pin.n = n
-        for k in ("x", "y", "z", "vx", "vy", "vz", "ax", "ay", "az", "density", "mass", "temperature"):
-            setattr(pin, k, arr(k))
-        ft = particles.get("fluid_type")
-        if ft is not None:
-            fta = np.ascontiguousarray(ft, dtype=np.uint8)
-            keep.append(fta)
-            pin.fluid_type = fta.ctypes.data_as(C.POINTER(C.c_uint8))
+        pin = _particles_in(particles, keep)
         bin_ = None
         if boundary is not None and len(boundary["x"]) > 0:
             bin_ = _BoundaryIn()
@@ -271,6 +316,31 @@ class HipKernel:
             raise ValueError("workgroup_size must be 32, 64, 128, or 256")
         self.set_tuning(max(wg, 64), 0)
 
+    def autotune(self, trial_steps: int = 5) -> dict:
+        """Pick the list capacity and per-pass workgroup sizes from timed trial
+        steps; the simulation state is restored (it does not advance)."""
+        t = _Tuning()
+        _check(lib().ubp_autotune(self._h, int(trial_steps), C.byref(t)), "autotune")
+        return _tuning_dict(t)
+
+    def get_tuning(self) -> dict:
+        t = _Tuning()
+        _check(lib().ubp_get_tuning(self._h, C.byref(t)), "get_tuning")
+        return _tuning_dict(t)
+
+    def set_pass_blocks(self, blocks, nbr_capacity: int = 0):
+        """Workgroup size per pass (sequence of NPASS or dict keyed by PASSES)."""
+        if isinstance(blocks, dict):
+            blocks = [blocks[p] for p in PASSES]
+        t = _Tuning()
+        for k, b in enumerate(blocks):
+            t.block[k] = int(b)
+        t.nbr_capacity = int(nbr_capacity)
+        _check(lib().ubp_set_tuning_ex(self._h, C.byref(t)), "set_tuning_ex")
+
+    def scene_count(self) -> int:
+        return int(lib().ubp_scene_count(self._h))
+
     # -- SimulationKernel ----------------------------------------------------
     def step(self, dt: float):
         _check(lib().ubp_step(self._h, float(dt)), "step")
@@ -324,10 +394,14 @@ class HipKernel:
         return {k: int(getattr(p, k)) for k, _ in _Profile._fields_}
 
     def step_n_profiled(self, nsteps: int) -> dict:
-        """n device-dt steps; summed per-kernel milliseconds (HIP events)."""
+        """n device-dt steps; summed per-kernel milliseconds (HIP events) over
+        the timed steps (all of them once the engine has stepped; the first
+        step of a fresh engine runs untimed).  ``timed_steps`` holds the count."""
         out = np.zeros(len(KERNELS), np.float64)
-        _check(lib().ubp_step_n_profiled(self._h, int(nsteps), out.ctypes.data_as(C.POINTER(C.c_double))),
-               "step_n_profiled")
+        timed = _check(lib().ubp_step_n_profiled(self._h, int(nsteps),
+                                                 out.ctypes.data_as(C.POINTER(C.c_double))),
+                       "step_n_profiled")
+        self.timed_steps = int(timed)
         return dict(zip(KERNELS, out.tolist()))
 
     def debug_bins(self):
@@ -343,6 +417,80 @@ class HipKernel:
         b = np.zeros(3, np.uint32)
         _check(lib().ubp_grid_info(self._h, _u32p(d), _u32p(b)), "grid_info")
         return d, b
+
+
+class HipBatch(HipKernel):
+    """Many independent scenes on one engine: every step advances each scene
+    by one step with its own CFL dt, all scenes in the same launches.  Each
+    scene's results are bit-identical to a ``HipKernel`` of that scene.
+
+    ``scenes``: list of scene dicts as taken by ``HipKernel.from_scene``
+    (particles, h, gravity, speed_of_sound, cfl_number, domain bounds,
+    alpha/beta, optional SDF); boundary particles are not supported.
+    """
+
+    def __init__(self, scenes: list):
+        L = lib()
+        keep = []
+        arr = (_Scene * len(scenes))()
+        self._offsets = [0]
+        for s, sc in enumerate(scenes):
+            if sc.get("boundary") is not None and len(sc["boundary"]["x"]) > 0:
+                raise GpuInitError("a batch does not take boundary particles")
+            pin = _particles_in(sc, keep)
+            keep.append(pin)
+            d = arr[s]
+            d.particles = C.pointer(pin)
+            d.h = float(sc["h"])
+            d.gravity[:] = [float(v) for v in sc["gravity"]]
+            d.speed_of_sound = float(sc["speed_of_sound"])
+            d.cfl_number = float(sc["cfl_number"])
+            d.domain_min[:] = [float(v) for v in sc["domain_min"]]
+            d.domain_max[:] = [float(v) for v in sc["domain_max"]]
+            d.alpha = float(sc.get("alpha", 1.0))
+            d.beta = float(sc.get("beta", 2.0))
+            sdf = sc.get("sdf")
+            if sdf is not None:
+                v = _f32(sdf["data"]).ravel()
+                dims = [int(x) for x in sdf["dims"]]
+                if v.size != dims[0] * dims[1] * dims[2]:
+                    raise GpuInitError("SDF data size does not match dims")
+                keep.append(v)
+                d.sdf = _fp(v)
+                d.sdf_origin[:] = [float(x) for x in sdf["origin"]]
+                d.sdf_cell = float(sdf["cell"])
+                d.sdf_dims[:] = dims
+            self._offsets.append(self._offsets[-1] + int(len(sc["x"])))
+        self._n = self._offsets[-1]
+        self._h = L.ubp_create_batch(arr, len(scenes))
+        if not self._h:
+            raise GpuInitError(_err())
+
+    def scene_range(self, s: int):
+        off = C.c_size_t(0)
+        cnt = C.c_size_t(0)
+        _check(lib().ubp_scene_range(self._h, int(s), C.byref(off), C.byref(cnt)), "scene_range")
+        return int(off.value), int(cnt.value)
+
+    def dts(self) -> np.ndarray:
+        """dt of every scene for the next device-dt step."""
+        v = np.zeros(self.scene_count(), np.float32)
+        _check(lib().ubp_get_scene_dts(self._h, _fp(v)), "get_scene_dts")
+        return v
+
+    def scene_particles(self, all_particles: dict | None = None) -> list:
+        """Per-scene views (original order within each scene)."""
+        p = self.particles() if all_particles is None else all_particles
+        out = []
+        for s in range(len(self._offsets) - 1):
+            a, b = self._offsets[s], self._offsets[s + 1]
+            out.append({k: v[a:b] for k, v in p.items()})
+        return out
+
+    def scene_error_metrics(self, s: int) -> ErrorMetrics:
+        m = _Metrics()
+        _check(lib().ubp_scene_error_metrics(self._h, int(s), C.byref(m)), "scene_error_metrics")
+        return ErrorMetrics(m.max_density_variation, m.energy_conservation, m.mass_conservation)
 
 
 def calibration_copy(nbytes: int, reps: int = 20) -> float:

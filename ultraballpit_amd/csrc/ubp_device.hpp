@@ -22,6 +22,7 @@ struct DevParams {
   float cell;          // grid cell = 2h (kernel/src/lib.rs:149)
   uint32_t dims[3];    // grid dims (neighbor.rs:31-35)
   uint32_t ncodes;     // Morton code space = 2^(bx+by+bz)
+  uint32_t code_base;  // first code of this scene in the shared code space
   float rsq;           // (2h)^2, neighbor.rs:144
   float eta_sq;        // 0.01*h*h, sph.rs:177/442
   float norm_h3;       // WENDLAND_C2_NORM_3D / h^3   (= W(0,h))

@@ -68,14 +68,109 @@ template <class T> static void dfree(T*& p) {
   p = nullptr;
 }
 
+// Morton grid of one scene (neighbor.rs:29-46 dims; cell = 2h, lib.rs:149).
+static void grid_params(DevParams& Q, uint32_t bits[3], float h, const float g[3], float cs, float cfl,
+                        const float dmin[3], const float dmax[3]) {
+  if (!(h > 0.0f)) throw Error(UBP_EINVAL, "h must be positive");
+  Q.h = h;
+  Q.inv_h = 1.0f / h;
+  Q.cs = cs;
+  Q.cfl = cfl;
+  for (int a = 0; a < 3; ++a) { Q.g[a] = g[a]; Q.dmin[a] = dmin[a]; Q.dmax[a] = dmax[a]; }
+  Q.cell = 2.0f * h;
+  uint64_t codes = 1;
+  for (int a = 0; a < 3; ++a) {
+    float d = std::ceil((dmax[a] - dmin[a]) / Q.cell);  // neighbor.rs:31-35
+    if (!(d >= 1.0f)) d = 1.0f;
+    if (d > 65536.0f) throw Error(UBP_EOVERFLOW, "grid too large");
+    Q.dims[a] = uint32_t(d);
+    bits[a] = bits_for(Q.dims[a]);
+    codes <<= bits[a];
+  }
+  if (codes > (uint64_t(1) << 28)) throw Error(UBP_EOVERFLOW, "Morton code space exceeds 2^28 cells");
+  Q.ncodes = uint32_t(codes);
+  Q.code_base = 0;
+  Q.diff_floor = 0.0f;
+  Q.sdf_on = 0;
+  Q.nb = 0;
+}
+
+// Constants derived from h, c_s, alpha, beta (sph.rs:20, :169-216, :432-492,
+// boundary.rs:143-181, eos.rs:39-47).
+static void set_derived(DevParams& Q, float alpha, float beta) {
+  const float h = Q.h;
+  const float pi_f = 3.14159265358979323846f;
+  const float sixteen_pi = 16.0f * pi_f;
+  const float wn = 21.0f / sixteen_pi;             // sph.rs:20
+  const float h3 = h * h * h;
+  Q.norm_h3 = wn / h3;
+  Q.norm_h4 = wn / (h3 * h);
+  const float sr = 2.0f * h;
+  Q.rsq = sr * sr;                                 // neighbor.rs:144
+  Q.eta_sq = 0.01f * h * h;
+  Q.diff_coef = 0.1f * h * Q.cs;
+  Q.nac = -alpha * Q.cs;
+  Q.beta = beta;
+  Q.r0 = 0.5f * h;
+  Q.d_rep = 10.0f * 9.81f * 0.01f;
+  const float rho0[2] = {1000.0f, 1.204f};
+  const float gam[2] = {7.0f, 1.4f};
+  for (int f = 0; f < 2; ++f) {
+    Q.rho0[f] = rho0[f];
+    Q.gamma[f] = gam[f];
+    Q.b_tait[f] = rho0[f] * Q.cs * Q.cs / gam[f];  // eos.rs:40
+  }
+}
+
+// Device copy of the evolving state (checkpoint and autotuner trials).
+struct Snapshot {
+  bool valid = false, needs_init = true;
+  int pp = 0;
+  float4 *pos = nullptr, *vel = nullptr, *acc = nullptr, *wall = nullptr;
+  float *rho = nullptr, *prs = nullptr;
+  uint8_t* ft = nullptr;
+  uint32_t *orig = nullptr, *start = nullptr;
+  uint16_t* sid = nullptr;
+  void release() {
+    dfree(pos); dfree(vel); dfree(acc); dfree(wall); dfree(rho); dfree(prs);
+    dfree(ft); dfree(orig); dfree(start); dfree(sid);
+    valid = false;
+  }
+};
+
+// Launches a kernel template in the engine's mode: many scenes (per-lane
+// parameters) or one scene (parameters in the argument block).
+#define UBP_LAUNCH_M(pass, kern, count, ...)                                     \
+  do {                                                                           \
+    if (multi) launch(pass, kern<true>, count, P, scn(), __VA_ARGS__);           \
+    else launch(pass, kern<false>, count, P, scn(), __VA_ARGS__);                \
+  } while (0)
+
+// Launch-shape passes the autotuner picks a workgroup size for.
+enum { PASS_ELEM = 0, PASS_NEIGH = 1, PASS_DENS = 2, PASS_FORCE = 3, PASS_XSPH = 4 };
+
 struct Engine {
   int dev = 0;
   hipStream_t st = nullptr;
   DevParams P{};
   size_t n = 0;
-  uint32_t block = 256;
+  uint32_t pblock[UBP_NPASS] = {256, 256, 256, 256, 256};
   uint32_t bits[3] = {0, 0, 0};
   uint32_t nblk_scan = 0;
+
+  // many-scene mode: scenes concatenated in one slot space, disjoint Morton
+  // code ranges, per-lane parameters and per-scene dt
+  uint32_t ns = 1;
+  bool multi = false;
+  std::vector<DevParams> hsp;
+  std::vector<uint32_t> sbits;  // 3 per scene
+  DevParams* d_sp = nullptr;
+  uint16_t* sid[2] = {nullptr, nullptr};
+  std::vector<uint32_t> soff{0, 0};
+  uint32_t* d_soff = nullptr;
+  uint32_t* sacc = nullptr;  // per-scene CFL maxima of straddling workgroups
+  std::vector<double> e0s, m0s;
+  Scenes scn() const { return Scenes{d_sp, sid[pp], ns}; }
 
   float4 *pos = nullptr, *pos_t = nullptr, *vel = nullptr, *vel_t = nullptr, *acc = nullptr;
   float4 *wall = nullptr, *wall_t = nullptr, *dv = nullptr;
@@ -98,14 +193,8 @@ struct Engine {
   float* bprs = nullptr;
   uint32_t *bl_start = nullptr, *bl = nullptr, *bc_start = nullptr, *bc = nullptr;
 
-  // checkpoint
-  bool has_ckpt = false;
-  bool ck_needs_init = true;
-  int ck_pp = 0;
-  float4 *ck_pos = nullptr, *ck_vel = nullptr, *ck_acc = nullptr, *ck_wall = nullptr;
-  float *ck_rho = nullptr, *ck_prs = nullptr;
-  uint8_t* ck_ft = nullptr;
-  uint32_t *ck_orig = nullptr, *ck_start = nullptr;
+  // checkpoint (reference save/restore_checkpoint) and autotuner snapshot
+  Snapshot ck, tune_ck;
 
   // host mirror (original order)
   std::vector<float> h_temp;
@@ -137,41 +226,20 @@ struct Engine {
     dfree(stage_orig); dfree(nbr); dfree(nn); dfree(nmask); dfree(spread); dfree(sdf); dfree(dtp); dfree(red);
     dfree(pmax); dfree(psum); dfree(dbg);
     dfree(bpos); dfree(bprs); dfree(bl_start); dfree(bl); dfree(bc_start); dfree(bc);
-    dfree(ck_pos); dfree(ck_vel); dfree(ck_acc); dfree(ck_wall); dfree(ck_rho); dfree(ck_prs);
-    dfree(ck_ft); dfree(ck_orig); dfree(ck_start);
+    dfree(d_sp); dfree(sid[0]); dfree(sid[1]); dfree(d_soff); dfree(sacc);
+    ck.release();
+    tune_ck.release();
     if (h_stage) { (void)hipHostFree(h_stage); h_stage = nullptr; }
     if (st) { (void)hipStreamDestroy(st); st = nullptr; }
   }
 
-  uint32_t grid_of(size_t count) const { return uint32_t((count + block - 1) / block); }
-
-  void set_derived(float alpha, float beta) {
-    const float h = P.h;
-    const float pi_f = 3.14159265358979323846f;
-    const float sixteen_pi = 16.0f * pi_f;
-    const float wn = 21.0f / sixteen_pi;             // sph.rs:20
-    const float h3 = h * h * h;
-    P.norm_h3 = wn / h3;
-    P.norm_h4 = wn / (h3 * h);
-    const float sr = 2.0f * h;
-    P.rsq = sr * sr;                                 // neighbor.rs:144
-    P.eta_sq = 0.01f * h * h;
-    P.diff_coef = 0.1f * h * P.cs;
-    P.nac = -alpha * P.cs;
-    P.beta = beta;
-    P.r0 = 0.5f * h;
-    P.d_rep = 10.0f * 9.81f * 0.01f;
-    const float rho0[2] = {1000.0f, 1.204f};
-    const float gam[2] = {7.0f, 1.4f};
-    for (int f = 0; f < 2; ++f) {
-      P.rho0[f] = rho0[f];
-      P.gamma[f] = gam[f];
-      P.b_tait[f] = rho0[f] * P.cs * P.cs / gam[f];  // eos.rs:40
-    }
+  uint32_t grid_of(size_t count, int pass = PASS_ELEM) const {
+    return uint32_t((count + pblock[pass] - 1) / pblock[pass]);
   }
 
-  void create(const ubp_particles_in* pin, const ubp_boundary_in* bin, float h, const float g[3],
-              float cs, float cfl, const float dmin[3], const float dmax[3]) {
+  void set_derived(float alpha, float beta) { ubp::set_derived(P, alpha, beta); }
+
+  void init_device() {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) throw Error(UBP_ENODEV, "no HIP device visible");
     HIPCHK(hipGetDevice(&dev));
@@ -179,72 +247,46 @@ struct Engine {
     HIPCHK(hipGetDeviceProperties(&prop, dev));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
       throw Error(UBP_ENODEV, std::string("device is ") + prop.gcnArchName + ", this build targets gfx950");
-    if (!(h > 0.0f)) throw Error(UBP_EINVAL, "h must be positive");
-    n = pin ? pin->n : 0;
-    if (n > (size_t(1) << 28)) throw Error(UBP_EOVERFLOW, "too many particles (max 2^28)");
     HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  }
 
-    P.n = uint32_t(n);
-    P.K = 128;
-    P.h = h;
-    P.inv_h = 1.0f / h;
-    P.cs = cs;
-    P.cfl = cfl;
-    for (int a = 0; a < 3; ++a) { P.g[a] = g[a]; P.dmin[a] = dmin[a]; P.dmax[a] = dmax[a]; }
-    P.cell = 2.0f * h;  // lib.rs:149
-    uint64_t codes = 1;
-    for (int a = 0; a < 3; ++a) {
-      float d = std::ceil((dmax[a] - dmin[a]) / P.cell);  // neighbor.rs:31-35
-      if (!(d >= 1.0f)) d = 1.0f;
-      if (d > 65536.0f) throw Error(UBP_EOVERFLOW, "grid too large");
-      P.dims[a] = uint32_t(d);
-      bits[a] = bits_for(P.dims[a]);
-      codes <<= bits[a];
-    }
-    if (codes > (uint64_t(1) << 28)) throw Error(UBP_EOVERFLOW, "Morton code space exceeds 2^28 cells");
-    P.ncodes = uint32_t(codes);
-    P.diff_floor = 0.0f;
-    P.sdf_on = 0;
-    set_derived(1.0f, 2.0f);  // reference alpha/beta (sph.rs:440-441)
-
-    // Morton spread tables
-    std::vector<uint32_t> sp(P.dims[0] + P.dims[1] + P.dims[2]);
-    for (uint32_t c = 0; c < P.dims[0]; ++c) sp[c] = morton_host(c, 0, 0, bits);
-    for (uint32_t c = 0; c < P.dims[1]; ++c) sp[P.dims[0] + c] = morton_host(0, c, 0, bits);
-    for (uint32_t c = 0; c < P.dims[2]; ++c) sp[P.dims[0] + P.dims[1] + c] = morton_host(0, 0, c, bits);
-    dalloc(&spread, sp.size());
-    HIPCHK(hipMemcpy(spread, sp.data(), sp.size() * 4, hipMemcpyHostToDevice));
-    P.spread = spread;
-
-    // host mirror
+  // copies scene particles into the host mirror at [off, off + count)
+  void load_mirror(const ubp_particles_in* pin, size_t off, size_t count) {
     const float* src[12] = {pin ? pin->x : nullptr, pin ? pin->y : nullptr, pin ? pin->z : nullptr,
                             pin ? pin->vx : nullptr, pin ? pin->vy : nullptr, pin ? pin->vz : nullptr,
                             pin ? pin->ax : nullptr, pin ? pin->ay : nullptr, pin ? pin->az : nullptr,
                             pin ? pin->density : nullptr, nullptr, pin ? pin->mass : nullptr};
-    if (n && (!pin->x || !pin->y || !pin->z || !pin->mass || !pin->density))
+    if (count && (!pin->x || !pin->y || !pin->z || !pin->mass || !pin->density))
       throw Error(UBP_EINVAL, "x, y, z, mass and density are required");
-    for (int k = 0; k < 12; ++k) {
-      h_init[k].assign(n, 0.0f);
-      if (src[k]) std::memcpy(h_init[k].data(), src[k], n * 4);
-    }
-    h_temp.assign(n, 293.15f);
-    if (pin && pin->temperature) std::memcpy(h_temp.data(), pin->temperature, n * 4);
-    h_ft.assign(n, 0);
-    if (pin && pin->fluid_type) std::memcpy(h_ft.data(), pin->fluid_type, n);
-    for (size_t i = 0; i < n; ++i)
+    for (int k = 0; k < 12; ++k)
+      if (src[k] && count) std::memcpy(h_init[k].data() + off, src[k], count * 4);
+    if (pin && pin->temperature && count) std::memcpy(h_temp.data() + off, pin->temperature, count * 4);
+    if (pin && pin->fluid_type && count) std::memcpy(h_ft.data() + off, pin->fluid_type, count);
+    for (size_t i = off; i < off + count; ++i)
       if (h_ft[i] > 1) throw Error(UBP_EINVAL, "fluid_type must be 0 (Water) or 1 (Air)");
+  }
 
-    // initial energy / mass (lib.rs:152-156), sequential FP64 in original order
-    e0 = 0.0; m0 = 0.0;
-    for (size_t i = 0; i < n; ++i) {
-      double m = h_init[11][i];
+  void alloc_mirror() {
+    for (int k = 0; k < 12; ++k) h_init[k].assign(n, 0.0f);
+    h_temp.assign(n, 293.15f);
+    h_ft.assign(n, 0);
+  }
+
+  // initial energy / mass of the mirror range (lib.rs:152-156), sequential
+  // FP64 in original order
+  void initial_sums(size_t lo, size_t hi, const float g[3], double& e, double& m) const {
+    e = 0.0; m = 0.0;
+    for (size_t i = lo; i < hi; ++i) {
+      double mi = h_init[11][i];
       double vx = h_init[3][i], vy = h_init[4][i], vz = h_init[5][i];
-      e0 += 0.5 * m * (vx * vx + vy * vy + vz * vz);
-      e0 -= m * (double(g[0]) * h_init[0][i] + double(g[1]) * h_init[1][i] + double(g[2]) * h_init[2][i]);
-      m0 += m;
+      e += 0.5 * mi * (vx * vx + vy * vy + vz * vz);
+      e -= mi * (double(g[0]) * h_init[0][i] + double(g[1]) * h_init[1][i] + double(g[2]) * h_init[2][i]);
+      m += mi;
     }
+  }
 
-    // device state
+  // device state for n particles over P.ncodes codes; uploads the mirror
+  void alloc_upload() {
     dalloc(&pos, n); dalloc(&pos_t, n); dalloc(&vel, n); dalloc(&vel_t, n); dalloc(&acc, n);
     dalloc(&wall, n); dalloc(&wall_t, n); dalloc(&dv, n);
     dalloc(&rho_n, n); dalloc(&prs, n);
@@ -255,12 +297,15 @@ struct Engine {
     nblk_scan = (P.ncodes + 1023u) / 1024u;
     dalloc(&bsum, nblk_scan);
     alloc_nbr();
-    dalloc(&dtp, 4);
-    dalloc(&red, 2 * ((n + 63) / 64 + 1));  // CFL partials, enough for 64-thread blocks
+    dalloc(&dtp, std::max<size_t>(4, ns));
+    // CFL partials per workgroup (at least 64 threads): (max signal, max |a|)
+    // for one scene, plus the workgroup's scene id for many scenes
+    const size_t nred = (multi ? 3 : 2) * ((n + 63) / 64 + 1);
+    dalloc(&red, nred);
     dalloc(&pmax, (n + 255) / 256 + 1); dalloc(&psum, 2 * ((n + 255) / 256 + 1));
     HIPCHK(hipMemset(cnt, 0, (size_t(P.ncodes) + 1) * 4));
-    HIPCHK(hipMemset(red, 0, 8 * ((n + 63) / 64 + 1)));
-    HIPCHK(hipMemset(dtp, 0, 16));
+    HIPCHK(hipMemset(red, 0, nred * 4));
+    HIPCHK(hipMemset(dtp, 0, std::max<size_t>(4, ns) * 4));
     uint32_t total = uint32_t(n);
     HIPCHK(hipMemcpy(start + P.ncodes, &total, 4, hipMemcpyHostToDevice));  // start[ncodes] = N
 
@@ -277,9 +322,135 @@ struct Engine {
     std::vector<uint32_t> iota(n);
     for (size_t i = 0; i < n; ++i) iota[i] = uint32_t(i);
     if (n) HIPCHK(hipMemcpy(orig[pp], iota.data(), n * 4, hipMemcpyHostToDevice));
+  }
+
+  void create(const ubp_particles_in* pin, const ubp_boundary_in* bin, float h, const float g[3],
+              float cs, float cfl, const float dmin[3], const float dmax[3]) {
+    init_device();
+    n = pin ? pin->n : 0;
+    if (n > (size_t(1) << 28)) throw Error(UBP_EOVERFLOW, "too many particles (max 2^28)");
+    grid_params(P, bits, h, g, cs, cfl, dmin, dmax);
+    P.n = uint32_t(n);
+    P.K = 128;
+    set_derived(1.0f, 2.0f);  // reference alpha/beta (sph.rs:440-441)
+
+    // Morton spread tables
+    std::vector<uint32_t> sp(P.dims[0] + P.dims[1] + P.dims[2]);
+    for (uint32_t c = 0; c < P.dims[0]; ++c) sp[c] = morton_host(c, 0, 0, bits);
+    for (uint32_t c = 0; c < P.dims[1]; ++c) sp[P.dims[0] + c] = morton_host(0, c, 0, bits);
+    for (uint32_t c = 0; c < P.dims[2]; ++c) sp[P.dims[0] + P.dims[1] + c] = morton_host(0, 0, c, bits);
+    dalloc(&spread, sp.size());
+    HIPCHK(hipMemcpy(spread, sp.data(), sp.size() * 4, hipMemcpyHostToDevice));
+    P.spread = spread;
+
+    alloc_mirror();
+    load_mirror(pin, 0, n);
+    initial_sums(0, n, g, e0, m0);
+    alloc_upload();
     P.nb = 0;
     if (bin && bin->n > 0) setup_boundary(bin);
     HIPCHK(hipDeviceSynchronize());
+  }
+
+  // Many independent scenes in one engine: each scene keeps its own grid,
+  // SDF, c_s, alpha/beta and dt; all scenes advance one step per step call.
+  void create_batch(const ubp_scene* sc, uint32_t count) {
+    if (!sc || count == 0) throw Error(UBP_EINVAL, "no scenes");
+    if (count > 65535u) throw Error(UBP_EOVERFLOW, "at most 65535 scenes");
+    init_device();
+    ns = count;
+    multi = true;
+    hsp.assign(ns, DevParams{});
+    sbits.assign(3 * size_t(ns), 0);
+    soff.assign(size_t(ns) + 1, 0);
+    uint64_t codes = 0;
+    size_t nspread = 0, nsdf = 0;
+    for (uint32_t s = 0; s < ns; ++s) {
+      const ubp_scene& c = sc[s];
+      const size_t ni = c.particles ? c.particles->n : 0;
+      soff[s + 1] = soff[s] + uint32_t(std::min<size_t>(ni, size_t(1) << 28));
+      if (size_t(soff[s]) + ni > (size_t(1) << 28)) throw Error(UBP_EOVERFLOW, "too many particles (max 2^28)");
+      DevParams& Q = hsp[s];
+      grid_params(Q, &sbits[3 * size_t(s)], c.h, c.gravity, c.speed_of_sound, c.cfl_number, c.domain_min,
+                  c.domain_max);
+      ubp::set_derived(Q, c.alpha, c.beta);
+      Q.code_base = uint32_t(codes);
+      codes += Q.ncodes;
+      if (codes > (uint64_t(1) << 28)) throw Error(UBP_EOVERFLOW, "total Morton code space exceeds 2^28 cells");
+      nspread += size_t(Q.dims[0]) + Q.dims[1] + Q.dims[2];
+      if (c.sdf) {
+        if (!(c.sdf_cell > 0.0f) || c.sdf_dims[0] < 2 || c.sdf_dims[1] < 2 || c.sdf_dims[2] < 2)
+          throw Error(UBP_EINVAL, "invalid SDF grid");
+        nsdf += size_t(c.sdf_dims[0]) * c.sdf_dims[1] * c.sdf_dims[2];
+      }
+    }
+    n = soff[ns];
+    if (n == 0) throw Error(UBP_EINVAL, "the scenes hold no particles");
+    // shared launch parameters: totals
+    P = hsp[0];
+    P.n = uint32_t(n);
+    P.K = 128;
+    P.ncodes = uint32_t(codes);
+    P.code_base = 0;
+    for (int a = 0; a < 3; ++a) bits[a] = sbits[a];
+
+    std::vector<uint32_t> sp(nspread);
+    std::vector<float> sd(nsdf);
+    dalloc(&spread, nspread);
+    dalloc(&sdf, nsdf);
+    size_t os = 0, od = 0;
+    for (uint32_t s = 0; s < ns; ++s) {
+      DevParams& Q = hsp[s];
+      const uint32_t* bq = &sbits[3 * size_t(s)];
+      for (uint32_t c = 0; c < Q.dims[0]; ++c) sp[os + c] = morton_host(c, 0, 0, bq);
+      for (uint32_t c = 0; c < Q.dims[1]; ++c) sp[os + Q.dims[0] + c] = morton_host(0, c, 0, bq);
+      for (uint32_t c = 0; c < Q.dims[2]; ++c) sp[os + Q.dims[0] + Q.dims[1] + c] = morton_host(0, 0, c, bq);
+      Q.spread = spread + os;
+      os += size_t(Q.dims[0]) + Q.dims[1] + Q.dims[2];
+      if (sc[s].sdf) {
+        const size_t m = size_t(sc[s].sdf_dims[0]) * sc[s].sdf_dims[1] * sc[s].sdf_dims[2];
+        std::memcpy(sd.data() + od, sc[s].sdf, m * 4);
+        for (int a = 0; a < 3; ++a) { Q.so[a] = sc[s].sdf_origin[a]; Q.sd[a] = sc[s].sdf_dims[a]; }
+        Q.scell = sc[s].sdf_cell;
+        Q.sdf = sdf + od;
+        Q.sdf_on = 1;
+        od += m;
+      }
+      Q.n = uint32_t(n);
+      Q.K = P.K;
+    }
+    HIPCHK(hipMemcpy(spread, sp.data(), nspread * 4, hipMemcpyHostToDevice));
+    if (nsdf) HIPCHK(hipMemcpy(sdf, sd.data(), nsdf * 4, hipMemcpyHostToDevice));
+
+    alloc_mirror();
+    e0s.assign(ns, 0.0);
+    m0s.assign(ns, 0.0);
+    for (uint32_t s = 0; s < ns; ++s) {
+      load_mirror(sc[s].particles, soff[s], soff[s + 1] - soff[s]);
+      initial_sums(soff[s], soff[s + 1], sc[s].gravity, e0s[s], m0s[s]);
+    }
+    e0 = e0s[0];
+    m0 = m0s[0];
+    alloc_upload();
+    dalloc(&d_sp, ns);
+    upload_scene_params();
+    dalloc(&sid[0], n);
+    dalloc(&sid[1], n);
+    std::vector<uint16_t> hs(n);
+    for (uint32_t s = 0; s < ns; ++s)
+      for (uint32_t i = soff[s]; i < soff[s + 1]; ++i) hs[i] = uint16_t(s);
+    HIPCHK(hipMemcpy(sid[pp], hs.data(), n * 2, hipMemcpyHostToDevice));
+    dalloc(&d_soff, size_t(ns) + 1);
+    HIPCHK(hipMemcpy(d_soff, soff.data(), (size_t(ns) + 1) * 4, hipMemcpyHostToDevice));
+    dalloc(&sacc, 2 * size_t(ns));
+    HIPCHK(hipMemset(sacc, 0, 2 * size_t(ns) * 4));
+    HIPCHK(hipDeviceSynchronize());
+  }
+
+  void upload_scene_params() {
+    for (auto& Q : hsp) Q.K = P.K;
+    HIPCHK(hipMemcpyAsync(d_sp, hsp.data(), hsp.size() * sizeof(DevParams), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
   }
 
   // Static candidate lists for the boundary particles (boundary.rs:20-37).
@@ -373,11 +544,18 @@ struct Engine {
   }
 
   // neighbour masks of the current positions, then the flat lists
-  void build_neighbors() {
-    launch(k_neigh, grid_of(n), P, pos, start, nmask, nn);
+  void build_neighbors(hipEvent_t* E = nullptr) {
+    UBP_LAUNCH_M(PASS_NEIGH, k_neigh, n, pos, start, nmask, nn);
+    rec(E, 7);
     if (n) {
-      hipLaunchKernelGGL(k_expand, dim3(uint32_t((n + 63) / 64)), dim3(64), 0, st, P, (const float4*)pos,
-                         (const uint32_t*)start, (const uint32_t*)nmask, (const uint32_t*)nn, nbr);
+      if (multi)
+        hipLaunchKernelGGL(k_expand<true>, dim3(uint32_t((n + 63) / 64)), dim3(64), 0, st, P, scn(),
+                           (const float4*)pos, (const uint32_t*)start, (const uint32_t*)nmask,
+                           (const uint32_t*)nn, nbr);
+      else
+        hipLaunchKernelGGL(k_expand<false>, dim3(uint32_t((n + 63) / 64)), dim3(64), 0, st, P, scn(),
+                           (const float4*)pos, (const uint32_t*)start, (const uint32_t*)nmask,
+                           (const uint32_t*)nn, nbr);
       LAUNCHCHK("k_expand");
     }
   }
@@ -393,6 +571,7 @@ struct Engine {
 
   void set_sdf(const float o[3], float cellsz, const uint32_t d[3], const float* data) {
     check_not_started("ubp_set_sdf");
+    if (multi) throw Error(UBP_EINVAL, "a batch engine takes each scene's SDF at creation");
     if (!(cellsz > 0.0f) || d[0] < 2 || d[1] < 2 || d[2] < 2 || !data)
       throw Error(UBP_EINVAL, "invalid SDF grid");
     size_t nn_ = size_t(d[0]) * d[1] * d[2];
@@ -409,28 +588,44 @@ struct Engine {
 
   void set_visc(float alpha, float beta) {
     check_not_started("ubp_set_viscosity_coeffs");
+    if (multi) throw Error(UBP_EINVAL, "a batch engine takes each scene's alpha/beta at creation");
     set_derived(alpha, beta);
     invalidate_graph();
   }
 
+  static bool valid_block(uint32_t b) { return b == 64 || b == 128 || b == 256; }
+
   void set_tuning(uint32_t bs, uint32_t K) {
     if (bs) {
-      if (bs != 64 && bs != 128 && bs != 256) throw Error(UBP_EINVAL, "block size must be 64, 128 or 256");
-      block = bs;
+      if (!valid_block(bs)) throw Error(UBP_EINVAL, "block size must be 64, 128 or 256");
+      for (auto& b : pblock) b = bs;
     }
+    set_capacity(K);
+    invalidate_graph();
+  }
+
+  void set_capacity(uint32_t K) {
     if (K && K != P.K) {
       if (K < 1 || K > 1024) throw Error(UBP_EINVAL, "neighbour capacity must be in [1, 1024]");
       HIPCHK(hipStreamSynchronize(st));
       P.K = K;
+      if (multi) upload_scene_params();
       alloc_nbr();
       if (!needs_init) build_neighbors();
+      invalidate_graph();
     }
+  }
+
+  void set_pass_blocks(const uint32_t* b) {
+    for (int k = 0; k < UBP_NPASS; ++k)
+      if (!valid_block(b[k])) throw Error(UBP_EINVAL, "block size must be 64, 128 or 256");
+    for (int k = 0; k < UBP_NPASS; ++k) pblock[k] = b[k];
     invalidate_graph();
   }
 
-  template <class Kern, class... A> void launch(Kern k, uint32_t grid, A... args) {
-    if (grid == 0) return;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(block), 0, st, args...);
+  template <class Kern, class... A> void launch(int pass, Kern k, size_t count, A... args) {
+    if (count == 0) return;
+    hipLaunchKernelGGL(k, dim3(grid_of(count, pass)), dim3(pblock[pass]), 0, st, args...);
     LAUNCHCHK("kernel launch");
   }
 
@@ -451,58 +646,64 @@ struct Engine {
   // counting sort of pos_t order into slot order (keys/arrivals already made)
   void bin(hipEvent_t* E = nullptr) {
     scan();
-    rec(E, 3);
-    const uint32_t g = grid_of(n);
-    launch(k_scatter, g, P, key, arr, start, orig[pp], stage_src, stage_orig);
     rec(E, 4);
-    launch(k_rank_gather, g, P, stage_src, stage_orig, key, start, pos_t, vel_t, rho_n, wall_t, ft[pp],
-           pos, vel, dr, wall, ft[pp ^ 1], orig[pp ^ 1]);
+    launch(PASS_ELEM, k_scatter, n, P, key, arr, start, orig[pp], stage_src, stage_orig);
+    rec(E, 5);
+    UBP_LAUNCH_M(PASS_ELEM, k_rank_gather, n, stage_src, stage_orig, key, start, pos_t, vel_t, rho_n, wall_t,
+                 ft[pp], pos, vel, dr, wall, ft[pp ^ 1], orig[pp ^ 1], sid[pp], sid[pp ^ 1]);
     pp ^= 1;
   }
 
   void density_forces(hipEvent_t* E = nullptr) {
-    const uint32_t g = grid_of(n);
-    rec(E, 5);
-    build_neighbors();
     rec(E, 6);
-    launch(k_density, g, P, dr, vel, wall, ft[pp], start, nbr, nn, dtp, rho_n, prs, fr);
-    if (P.nb) launch(k_adami, grid_of(P.nb), P, pos, rho_n, prs, orig[pp], start, bprs);
-    rec(E, 7);
-    launch(k_forces, g, P, fr, wall, ft[pp], start, nbr, nn, acc);
+    build_neighbors(E);
     rec(E, 8);
+    UBP_LAUNCH_M(PASS_DENS, k_density, n, dr, vel, wall, ft[pp], start, nbr, nn, dtp, rho_n, prs, fr);
+    if (P.nb) launch(PASS_ELEM, k_adami, P.nb, P, pos, rho_n, prs, orig[pp], start, bprs);
+    rec(E, 9);
+    UBP_LAUNCH_M(PASS_FORCE, k_forces, n, fr, wall, ft[pp], start, nbr, nn, acc);
+    rec(E, 10);
   }
 
   // CpuKernel::compute_forces on the initial state (lib.rs:295-298)
   void bootstrap() {
-    launch(k_key_count, grid_of(n), P, pos_t, key, arr, cnt, wall_t);
+    UBP_LAUNCH_M(PASS_ELEM, k_key_count, n, pos_t, key, arr, cnt, wall_t);
     bin();
     density_forces();
     needs_init = false;
   }
 
-  // lib.rs:300-334 with the dt on the device; E (10 events) brackets kernels
+  // lib.rs:300-334 with the dt on the device; E (UBP_NKERN + 1 events)
+  // brackets the kernels in the order documented for ubp_step_n_profiled
   void step_body(int do_cfl, hipEvent_t* E = nullptr) {
-    const uint32_t g = grid_of(n);
     rec(E, 0);
-    launch(k_kick, g, P, vel, acc, pos, rho_n, dtp, xr);
+    UBP_LAUNCH_M(PASS_ELEM, k_kick, n, vel, acc, pos, rho_n, dtp, xr);
     rec(E, 1);
-    launch(k_xsph, g, P, xr, start, nbr, nn, dv);
-    launch(k_drift, g, P, pos, vel, dv, dtp, pos_t, vel_t, wall_t, key, arr, cnt);
+    UBP_LAUNCH_M(PASS_XSPH, k_xsph, n, xr, start, nbr, nn, dv);
     rec(E, 2);
+    UBP_LAUNCH_M(PASS_ELEM, k_drift, n, pos, vel, dv, dtp, pos_t, vel_t, wall_t, key, arr, cnt);
+    rec(E, 3);
     bin(E);
     density_forces(E);
-    launch(k_kick2_cfl, g, P, vel, acc, dtp, red, 1, do_cfl);
-    if (do_cfl) cfl_finalize();
-    rec(E, 9);
+    UBP_LAUNCH_M(PASS_ELEM, k_kick2_cfl, n, vel, acc, dtp, red, sacc, 1, do_cfl);
+    if (do_cfl) cfl_finalize(true);
+    rec(E, 11);
   }
 
-  void cfl_finalize() {
-    hipLaunchKernelGGL(k_cfl_finalize, dim3(1), dim3(256), 0, st, P, (const uint32_t*)red, grid_of(n), dtp);
+  // dt from the CFL partials; `sorted`: slots are binned (scene ranges from
+  // start[]) rather than in input order (scene ranges from the offsets)
+  void cfl_finalize(bool sorted) {
+    if (multi)
+      hipLaunchKernelGGL(k_cfl_finalize_scenes, dim3(ns), dim3(64), 0, st, (const DevParams*)d_sp,
+                         (const uint32_t*)red, sacc, (const uint32_t*)start, (const uint32_t*)d_soff,
+                         sorted ? 1 : 0, pblock[PASS_ELEM], ns, dtp);
+    else
+      hipLaunchKernelGGL(k_cfl_finalize, dim3(1), dim3(256), 0, st, P, (const uint32_t*)red, grid_of(n), dtp);
     LAUNCHCHK("k_cfl_finalize");
   }
 
   void set_dt(float dt) {
-    hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(64), 0, st, dtp, dt);
+    hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(64), 0, st, dtp, dt, ns);
     LAUNCHCHK("k_set_dt");
   }
 
@@ -518,13 +719,9 @@ struct Engine {
   void ensure_dt() {
     if (dt_valid) return;
     // compute_timestep on the current state (initial arrays before bootstrap)
-    const float4* v = needs_init ? vel_t : vel;
-    if (n > 0) {
-      hipLaunchKernelGGL(k_kick2_cfl, dim3(grid_of(n)), dim3(block), 0, st, P, (float4*)v, (const float4*)acc,
-                         (const float*)dtp, red, 0, 1);
-      LAUNCHCHK("k_kick2_cfl(cfl only)");
-    }
-    cfl_finalize();  // with no particles: max signal = c_s, no force limit
+    float4* v = needs_init ? vel_t : vel;
+    UBP_LAUNCH_M(PASS_ELEM, k_kick2_cfl, n, v, (const float4*)acc, (const float*)dtp, red, sacc, 0, 1);
+    cfl_finalize(!needs_init);  // with no particles: max signal = c_s, no force limit
     dt_valid = true;
   }
 
@@ -569,14 +766,16 @@ struct Engine {
 
   void sync() { HIPCHK(hipStreamSynchronize(st)); }
 
-  // device-dt steps with HIP events around every launch (eager launches)
-  void step_n_profiled(int nsteps, double* kernel_ms) {
+  // device-dt steps with HIP events around every launch (eager launches);
+  // returns the number of steps timed (the first step of a fresh engine,
+  // which carries the bootstrap force pass, is run untimed)
+  int step_n_profiled(int nsteps, double* kernel_ms) {
     for (int k = 0; k < UBP_NKERN; ++k) kernel_ms[k] = 0.0;
-    if (n == 0 || nsteps <= 0) return;
+    if (n == 0 || nsteps <= 0) return 0;
     stepped = true;
     ensure_dt();
     if (needs_init) { bootstrap(); step_body(1); --nsteps; }
-    if (nsteps <= 0) { sync(); return; }
+    if (nsteps <= 0) { sync(); return 0; }
     const int per = UBP_NKERN + 1;
     std::vector<hipEvent_t> ev(size_t(nsteps) * per, nullptr);
     try {
@@ -596,6 +795,7 @@ struct Engine {
       throw;
     }
     for (auto& x : ev) (void)hipEventDestroy(x);
+    return nsteps;
   }
 
   void ensure_stage(size_t bytes) {
@@ -654,12 +854,27 @@ struct Engine {
     if (o->fluid_type) std::memcpy(o->fluid_type, h_ft.data(), n);
   }
 
-  void metrics(ubp_error_metrics_t* out) {
-    if (n == 0) { out->max_density_variation = 0; out->energy_conservation = 0; out->mass_conservation = 0; return; }
-    const uint32_t g = uint32_t((n + 255) / 256);  // k_metrics is written for 256 threads
+  // slot range [lo, hi) of scene s in the current order
+  void scene_range(uint32_t s, uint32_t& lo, uint32_t& hi) {
+    if (!multi) { lo = 0; hi = uint32_t(n); return; }
+    if (needs_init) { lo = soff[s]; hi = soff[s + 1]; return; }
+    const DevParams& Q = hsp[s];
+    HIPCHK(hipMemcpyAsync(&lo, start + Q.code_base, 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&hi, start + Q.code_base + Q.ncodes, 4, hipMemcpyDeviceToHost, st));
+    sync();
+  }
+
+  void metrics(ubp_error_metrics_t* out, uint32_t s = 0) {
+    if (s >= ns) throw Error(UBP_EINVAL, "scene index out of range");
+    uint32_t lo = 0, hi = 0;
+    if (n) scene_range(s, lo, hi);
+    if (hi <= lo) { out->max_density_variation = 0; out->energy_conservation = 0; out->mass_conservation = 0; return; }
+    const DevParams& Q = multi ? hsp[s] : P;
+    const double e0_ = multi ? e0s[s] : e0, m0_ = multi ? m0s[s] : m0;
+    const uint32_t g = uint32_t((hi - lo + 255) / 256);  // k_metrics is written for 256 threads
     const float4* p = needs_init ? pos_t : pos;
     const float4* v = needs_init ? vel_t : vel;
-    hipLaunchKernelGGL(k_metrics, dim3(g), dim3(256), 0, st, P, p, v, (const float*)rho_n,
+    hipLaunchKernelGGL(k_metrics, dim3(g), dim3(256), 0, st, Q, lo, hi, p, v, (const float*)rho_n,
                        (const uint8_t*)ft[pp], pmax, psum);
     LAUNCHCHK("k_metrics");
     std::vector<float> hm(g);
@@ -675,18 +890,19 @@ struct Engine {
       m += hs[2 * b + 1];
     }
     out->max_density_variation = mx;
-    out->energy_conservation = (std::fabs(e0) > 1.0e-12) ? float(std::fabs((e - e0) / e0)) : float(std::fabs(e - e0));
-    out->mass_conservation = (std::fabs(m0) > 1.0e-12) ? float(std::fabs((m - m0) / m0)) : float(std::fabs(m - m0));
+    out->energy_conservation =
+        (std::fabs(e0_) > 1.0e-12) ? float(std::fabs((e - e0_) / e0_)) : float(std::fabs(e - e0_));
+    out->mass_conservation =
+        (std::fabs(m0_) > 1.0e-12) ? float(std::fabs((m - m0_) / m0_)) : float(std::fabs(m - m0_));
   }
 
   void debug_bins(uint32_t* keys, uint32_t* perm, uint32_t* counts) {
     if (needs_init) throw Error(UBP_EINVAL, "no binning before the first step");
     if (!dbg) dalloc(&dbg, n);
-    const uint32_t g = grid_of(n);
     std::vector<uint32_t> ho(n), hv(n);
     HIPCHK(hipMemcpyAsync(ho.data(), orig[pp], n * 4, hipMemcpyDeviceToHost, st));
     if (keys) {
-      launch(k_debug_keys, g, P, pos, orig[pp], dbg);
+      UBP_LAUNCH_M(PASS_ELEM, k_debug_keys, n, pos, orig[pp], dbg);
       HIPCHK(hipMemcpyAsync(keys, dbg, n * 4, hipMemcpyDeviceToHost, st));
     }
     if (counts) HIPCHK(hipMemcpyAsync(hv.data(), nn, n * 4, hipMemcpyDeviceToHost, st));
@@ -696,48 +912,118 @@ struct Engine {
       for (size_t s = 0; s < n; ++s) counts[ho[s]] = hv[s];
   }
 
-  void save_ckpt() {
-    if (!ck_pos) {
-      dalloc(&ck_pos, n); dalloc(&ck_vel, n); dalloc(&ck_acc, n); dalloc(&ck_wall, n);
-      dalloc(&ck_rho, n); dalloc(&ck_prs, n);
-      dalloc(&ck_ft, n); dalloc(&ck_orig, n); dalloc(&ck_start, size_t(P.ncodes) + 1);
+  void save_snap(Snapshot& S) {
+    if (!S.pos) {
+      dalloc(&S.pos, n); dalloc(&S.vel, n); dalloc(&S.acc, n); dalloc(&S.wall, n);
+      dalloc(&S.rho, n); dalloc(&S.prs, n);
+      dalloc(&S.ft, n); dalloc(&S.orig, n); dalloc(&S.start, size_t(P.ncodes) + 1);
+      if (multi) dalloc(&S.sid, n);
     }
     auto cp = [&](void* d, const void* s, size_t b) { if (b) HIPCHK(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, st)); };
     const bool pre = needs_init;
-    cp(ck_pos, pre ? pos_t : pos, n * 16);
-    cp(ck_vel, pre ? vel_t : vel, n * 16);
-    cp(ck_acc, acc, n * 16);
-    cp(ck_wall, wall, n * 16);
-    cp(ck_rho, rho_n, n * 4);
-    cp(ck_prs, prs, n * 4);
-    cp(ck_ft, ft[pp], n);
-    cp(ck_orig, orig[pp], n * 4);
-    cp(ck_start, start, (size_t(P.ncodes) + 1) * 4);
-    ck_needs_init = needs_init;
-    ck_pp = pp;
-    has_ckpt = true;
+    cp(S.pos, pre ? pos_t : pos, n * 16);
+    cp(S.vel, pre ? vel_t : vel, n * 16);
+    cp(S.acc, acc, n * 16);
+    cp(S.wall, wall, n * 16);
+    cp(S.rho, rho_n, n * 4);
+    cp(S.prs, prs, n * 4);
+    cp(S.ft, ft[pp], n);
+    cp(S.orig, orig[pp], n * 4);
+    cp(S.start, start, (size_t(P.ncodes) + 1) * 4);
+    if (multi) cp(S.sid, sid[pp], n * 2);
+    S.needs_init = needs_init;
+    S.pp = pp;
+    S.valid = true;
   }
 
-  bool restore_ckpt() {
-    if (!has_ckpt) return false;
+  bool restore_snap(Snapshot& S) {
+    if (!S.valid) return false;
     auto cp = [&](void* d, const void* s, size_t b) { if (b) HIPCHK(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, st)); };
-    pp = ck_pp;
-    needs_init = ck_needs_init;
-    cp(needs_init ? pos_t : pos, ck_pos, n * 16);
-    cp(needs_init ? vel_t : vel, ck_vel, n * 16);
-    cp(acc, ck_acc, n * 16);
-    cp(wall, ck_wall, n * 16);
-    cp(rho_n, ck_rho, n * 4);
-    cp(prs, ck_prs, n * 4);
-    cp(ft[pp], ck_ft, n);
-    cp(orig[pp], ck_orig, n * 4);
-    cp(start, ck_start, (size_t(P.ncodes) + 1) * 4);
+    pp = S.pp;
+    needs_init = S.needs_init;
+    cp(needs_init ? pos_t : pos, S.pos, n * 16);
+    cp(needs_init ? vel_t : vel, S.vel, n * 16);
+    cp(acc, S.acc, n * 16);
+    cp(wall, S.wall, n * 16);
+    cp(rho_n, S.rho, n * 4);
+    cp(prs, S.prs, n * 4);
+    cp(ft[pp], S.ft, n);
+    cp(orig[pp], S.orig, n * 4);
+    cp(start, S.start, (size_t(P.ncodes) + 1) * 4);
+    if (multi) cp(sid[pp], S.sid, n * 2);
     if (!needs_init) build_neighbors();  // neighbour lists of the restored state
-    has_ckpt = false;
     dt_valid = false;
     invalidate_graph();
     sync();
     return true;
+  }
+
+  void save_ckpt() { save_snap(ck); }
+
+  bool restore_ckpt() {
+    if (!restore_snap(ck)) return false;
+    ck.valid = false;
+    return true;
+  }
+
+  // Autotuner: the largest neighbour count sets the list capacity (rounded
+  // up with 25% headroom), then every pass takes the workgroup size with the
+  // lowest event time over `trial` device-dt steps.  Each trial starts from
+  // the same snapshot and the state is restored at the end, so tuning never
+  // advances the simulation.  Every workgroup size gives the same bits.
+  void autotune(int trial, ubp_tuning* out) {
+    if (trial < 2) trial = 2;
+    if (n == 0) { report_tuning(out, 0.0); return; }
+    const bool was_stepped = stepped;
+    sync();
+    save_snap(tune_ck);
+    try {
+      // capacity from the neighbour counts of one step
+      step_device_dt(1, false);
+      sync();
+      std::vector<uint32_t> hn(n);
+      HIPCHK(hipMemcpy(hn.data(), nn, n * 4, hipMemcpyDeviceToHost));
+      uint32_t mx = 0;
+      for (uint32_t v : hn) mx = std::max(mx, v & ~UBP_NN_OVF);
+      uint32_t K = std::min<uint32_t>(1024u, std::max<uint32_t>(32u, ((mx + mx / 4u + 31u) / 32u) * 32u));
+      restore_snap(tune_ck);
+      set_capacity(K);
+      const uint32_t cand[3] = {64, 128, 256};
+      double best[UBP_NPASS];
+      uint32_t pick[UBP_NPASS];
+      for (int k = 0; k < UBP_NPASS; ++k) { best[k] = 1e300; pick[k] = pblock[k]; }
+      std::vector<double> ms(UBP_NKERN);
+      for (uint32_t c : cand) {
+        for (auto& b : pblock) b = c;
+        invalidate_graph();
+        step_n_profiled(trial, ms.data());
+        restore_snap(tune_ck);
+        // event brackets -> passes (kernel order of ubp_step_n_profiled)
+        const double t[UBP_NPASS] = {ms[0] + ms[2] + ms[4] + ms[5] + ms[10], ms[6], ms[8], ms[9], ms[1]};
+        for (int k = 0; k < UBP_NPASS; ++k)
+          if (t[k] < best[k]) { best[k] = t[k]; pick[k] = c; }
+      }
+      for (int k = 0; k < UBP_NPASS; ++k) pblock[k] = pick[k];
+      invalidate_graph();
+      const int timed = step_n_profiled(trial, ms.data());
+      restore_snap(tune_ck);
+      double tot = 0.0;
+      for (double v : ms) tot += v;
+      report_tuning(out, timed > 0 ? tot / timed : 0.0);
+    } catch (...) {
+      tune_ck.release();
+      stepped = was_stepped;
+      throw;
+    }
+    tune_ck.release();
+    stepped = was_stepped;
+  }
+
+  void report_tuning(ubp_tuning* out, double step_ms) const {
+    if (!out) return;
+    for (int k = 0; k < UBP_NPASS; ++k) out->block[k] = pblock[k];
+    out->nbr_capacity = P.K;
+    out->step_ms = step_ms;
   }
 };
 
@@ -895,13 +1181,13 @@ int ubp_step_profiled(ubp_engine* e, float dt, ubp_step_profile* out) {
   g.stepped = true;
   g.set_dt(dt);
   if (g.needs_init) { g.bootstrap(); }
-  hipEvent_t ev[10];
+  hipEvent_t ev[UBP_NKERN + 1];
   for (auto& x : ev) HIPCHK(hipEventCreate(&x));
   g.step_body(0, ev);
   g.sync();
   g.dt_valid = false;
-  float ms[9];
-  for (int k = 0; k < 9; ++k) HIPCHK(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
+  float ms[UBP_NKERN];
+  for (int k = 0; k < UBP_NKERN; ++k) HIPCHK(hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
   auto t1 = std::chrono::steady_clock::now();
   // readback leg of the reference profile: a full particles() copy
   {
@@ -915,10 +1201,10 @@ int ubp_step_profiled(ubp_engine* e, float dt, ubp_step_profile* out) {
   }
   auto t2 = std::chrono::steady_clock::now();
   for (auto& x : ev) (void)hipEventDestroy(x);
-  out->integrate_us = uint64_t((ms[0] + ms[1] + ms[8]) * 1000.0f);
-  out->grid_build_us = uint64_t((ms[2] + ms[3] + ms[4]) * 1000.0f);
-  out->density_us = uint64_t((ms[5] + ms[6]) * 1000.0f);
-  out->forces_us = uint64_t(ms[7] * 1000.0f);
+  out->integrate_us = uint64_t((ms[0] + ms[1] + ms[2] + ms[10]) * 1000.0f);
+  out->grid_build_us = uint64_t((ms[3] + ms[4] + ms[5]) * 1000.0f);
+  out->density_us = uint64_t((ms[6] + ms[7] + ms[8]) * 1000.0f);
+  out->forces_us = uint64_t(ms[9] * 1000.0f);
   out->boundary_pressure_us = 0;  // inside the density bracket (k_adami follows k_density)
   out->readback_us = uint64_t(std::chrono::duration<double, std::micro>(t2 - t1).count());
   out->total_us = uint64_t(std::chrono::duration<double, std::micro>(t2 - t0).count());
@@ -931,8 +1217,7 @@ int ubp_step_n_profiled(ubp_engine* e, int nsteps, double* kernel_ms) {
   UBP_HANDLE_CHECK(e);
   if (!kernel_ms) throw Error(UBP_EINVAL, "null kernel_ms");
   if (nsteps < 0) throw Error(UBP_EINVAL, "nsteps < 0");
-  e->eng.step_n_profiled(nsteps, kernel_ms);
-  return UBP_OK;
+  return e->eng.step_n_profiled(nsteps, kernel_ms);
   UBP_API_END
 }
 
@@ -1015,6 +1300,83 @@ int ubp_debug_bins(ubp_engine* e, uint32_t* keys, uint32_t* perm, uint32_t* nbr_
   UBP_API_BEGIN
   UBP_HANDLE_CHECK(e);
   e->eng.debug_bins(keys, perm, nbr_counts);
+  return UBP_OK;
+  UBP_API_END
+}
+
+ubp_engine* ubp_create_batch(const ubp_scene* scenes, uint32_t nscenes) {
+  ubp::g_last_error.clear();
+  ubp_engine* e = nullptr;
+  try {
+    e = new ubp_engine();
+    e->eng.create_batch(scenes, nscenes);
+    return e;
+  } catch (const Error& err) {
+    ubp::g_last_error = err.what();
+  } catch (const std::exception& err) {
+    ubp::g_last_error = err.what();
+  }
+  delete e;
+  return nullptr;
+}
+
+uint32_t ubp_scene_count(const ubp_engine* e) { return e ? e->eng.ns : 0u; }
+
+int ubp_scene_range(const ubp_engine* e, uint32_t s, size_t* offset, size_t* count) {
+  UBP_API_BEGIN
+  if (!e) throw Error(UBP_EINVAL, "null engine handle");
+  const Engine& g = e->eng;
+  if (s >= g.ns) throw Error(UBP_EINVAL, "scene index out of range");
+  const size_t lo = g.multi ? g.soff[s] : 0, hi = g.multi ? g.soff[s + 1] : g.n;
+  if (offset) *offset = lo;
+  if (count) *count = hi - lo;
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_get_scene_dts(ubp_engine* e, float* dts) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  if (!dts) throw Error(UBP_EINVAL, "null dts");
+  e->eng.ensure_dt();
+  HIPCHK(hipMemcpyAsync(dts, e->eng.dtp, size_t(e->eng.ns) * 4, hipMemcpyDeviceToHost, e->eng.st));
+  e->eng.sync();
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_scene_error_metrics(ubp_engine* e, uint32_t s, ubp_error_metrics_t* out) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  if (!out) throw Error(UBP_EINVAL, "null metrics");
+  e->eng.metrics(out, s);
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_autotune(ubp_engine* e, int trial_steps, ubp_tuning* out) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  e->eng.autotune(trial_steps, out);
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_get_tuning(const ubp_engine* e, ubp_tuning* out) {
+  UBP_API_BEGIN
+  if (!e) throw Error(UBP_EINVAL, "null engine handle");
+  if (!out) throw Error(UBP_EINVAL, "null tuning");
+  e->eng.report_tuning(out, 0.0);
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_set_tuning_ex(ubp_engine* e, const ubp_tuning* t) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  if (!t) throw Error(UBP_EINVAL, "null tuning");
+  e->eng.set_pass_blocks(t->block);
+  e->eng.set_capacity(t->nbr_capacity);
   return UBP_OK;
   UBP_API_END
 }

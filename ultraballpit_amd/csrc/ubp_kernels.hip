@@ -19,9 +19,22 @@ namespace ubp {
 
 #define UBP_LB __launch_bounds__(256)
 
+// Many-scene mode (template flag M): every slot carries its scene id; the
+// scene's parameters and dt are looked up per lane (each scene's DevParams
+// has n = K = the shared totals, so strides are unchanged).  With M = false
+// the parameters stay in the kernel argument P0 (SGPRs) and dt is dtp[0].
+struct Scenes {
+  const DevParams* sp;   // [nscenes] parameters
+  const uint16_t* sid;   // scene id per slot (current slot order)
+  uint32_t nscenes;
+};
+#define UBP_SCENE(i)                                         \
+  const uint32_t sc_ = M ? uint32_t(S.sid[(i)]) : 0u;        \
+  const DevParams& P = M ? S.sp[sc_] : P0
 
-__global__ void k_set_dt(float* dtp, float dt) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) dtp[0] = dt;
+
+__global__ void k_set_dt(float* dtp, float dt, uint32_t count) {
+  for (uint32_t k = threadIdx.x; k < count; k += blockDim.x) dtp[k] = dt;
 }
 
 // Arrival rank of this lane's particle in cell `code`: one atomic per
@@ -45,15 +58,17 @@ __device__ __forceinline__ uint32_t wave_histogram_add(uint32_t* __restrict__ cn
 
 // Cell key + histogram for positions in the current (unsorted) order; also the
 // SDF wall record at those positions.  Used for the bootstrap binning.
-__global__ UBP_LB void k_key_count(DevParams P, const float4* __restrict__ pos_t,
+template <bool M>
+__global__ UBP_LB void k_key_count(DevParams P0, Scenes S, const float4* __restrict__ pos_t,
                                    uint32_t* __restrict__ key, uint32_t* __restrict__ arr,
                                    uint32_t* __restrict__ cnt, float4* __restrict__ wall_t) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
+  if (i >= P0.n) return;
+  UBP_SCENE(i);
   float4 p = pos_t[i];
   uint32_t cx, cy, cz;
   pos_to_cell(P, p.x, p.y, p.z, cx, cy, cz);
-  uint32_t code = morton(P, cx, cy, cz);
+  uint32_t code = P.code_base + morton(P, cx, cy, cz);
   key[i] = code;
   arr[i] = wave_histogram_add(cnt, code);
   wall_t[i] = wall_at(P, p.x, p.y, p.z);
@@ -138,7 +153,8 @@ __global__ UBP_LB void k_scatter(DevParams P, const uint32_t* __restrict__ key,
 // Stable order: rank each staged particle within its cell by original id
 // (the reference visits a cell's particles in ascending index,
 // neighbor.rs:112-121), then gather the particle state into that slot.
-__global__ UBP_LB void k_rank_gather(DevParams P, const uint32_t* __restrict__ stage_src,
+template <bool M>
+__global__ UBP_LB void k_rank_gather(DevParams P0, Scenes S, const uint32_t* __restrict__ stage_src,
                                      const uint32_t* __restrict__ stage_orig,
                                      const uint32_t* __restrict__ key, const uint32_t* __restrict__ start,
                                      const float4* __restrict__ pos_t, const float4* __restrict__ vel_t,
@@ -146,9 +162,10 @@ __global__ UBP_LB void k_rank_gather(DevParams P, const uint32_t* __restrict__ s
                                      const uint8_t* __restrict__ ft_src,
                                      float4* __restrict__ pos, float4* __restrict__ vel,
                                      float4* __restrict__ dr, float4* __restrict__ wall,
-                                     uint8_t* __restrict__ ft_dst, uint32_t* __restrict__ orig_dst) {
+                                     uint8_t* __restrict__ ft_dst, uint32_t* __restrict__ orig_dst,
+                                     const uint16_t* __restrict__ sid_src, uint16_t* __restrict__ sid_dst) {
   uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
-  if (d >= P.n) return;
+  if (d >= P0.n) return;
   uint32_t s = stage_src[d];
   uint32_t c = key[s];
   uint32_t b = start[c], e = start[c + 1u];
@@ -156,6 +173,9 @@ __global__ UBP_LB void k_rank_gather(DevParams P, const uint32_t* __restrict__ s
   uint32_t rank = 0;
   for (uint32_t k = b; k < e; ++k) rank += (stage_orig[k] < me) ? 1u : 0u;
   uint32_t t = b + rank;
+  const uint32_t sc_ = M ? uint32_t(sid_src[s]) : 0u;
+  const DevParams& P = M ? S.sp[sc_] : P0;
+  if (M) sid_dst[t] = uint16_t(sc_);
   const float4 p = pos_t[s];
   const float rho = rho_src[s];
   // delta-SPH volume m_j / rho_j of the old density (sph.rs:199), computed
@@ -246,7 +266,7 @@ __device__ __forceinline__ void walk27r(const DevParams& P, uint32_t i, float4 p
                        (dy == 0 ? gy2[0] : (dy == 1 ? gy2[1] : gy2[2])) +
                        (dz == 0 ? gz2[0] : (dz == 1 ? gz2[1] : gz2[2]));
     if (gap2 > skip2) continue;
-    const uint32_t code = sx | sy | sz;
+    const uint32_t code = P.code_base + (sx | sy | sz);
     const uint32_t b = start[code], e = start[code + 1u];
     for (uint32_t s = b; s < e; ++s) {
       if (s == i) continue;
@@ -295,12 +315,14 @@ __device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, fl
   }
 }
 
-__global__ UBP_LB void k_neigh(DevParams P, const float4* __restrict__ pos,
+template <bool M>
+__global__ UBP_LB void k_neigh(DevParams P0, Scenes S, const float4* __restrict__ pos,
                                const uint32_t* __restrict__ start, uint32_t* __restrict__ nmask,
                                uint32_t* __restrict__ nn) {
   uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   uint32_t i = blk * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
+  if (i >= P0.n) return;
+  UBP_SCENE(i);
   const float4 pi = pos[i];
   uint32_t cx, cy, cz;
   pos_to_cell(P, pi.x, pi.y, pi.z, cx, cy, cz);
@@ -314,7 +336,7 @@ __global__ UBP_LB void k_neigh(DevParams P, const float4* __restrict__ pos,
     const uint32_t sx = sel3(N.sx, dx), sy = sel3(N.sy, dy), sz = sel3(N.sz, dz);
     uint32_t mask = 0u;
     if (sx != 0xffffffffu && sy != 0xffffffffu && sz != 0xffffffffu) {
-      const uint32_t code = sx | sy | sz;
+      const uint32_t code = P.code_base + (sx | sy | sz);
       const uint32_t b = start[code], e = start[code + 1u];
       if (e - b > 32u) ovf = UBP_NN_OVF;
       uint32_t bit = 1u;
@@ -342,7 +364,8 @@ __global__ UBP_LB void k_neigh(DevParams P, const float4* __restrict__ pos,
 // in a 32 x 64 LDS tile, and the wavefront then stores the 32 rows, 64
 // consecutive slots per row.
 #define UBP_EXPAND_CHUNK 32
-__global__ __launch_bounds__(64) void k_expand(DevParams P, const float4* __restrict__ pos,
+template <bool M>
+__global__ __launch_bounds__(64) void k_expand(DevParams P0, Scenes S, const float4* __restrict__ pos,
                                                const uint32_t* __restrict__ start,
                                                const uint32_t* __restrict__ nmask,
                                                const uint32_t* __restrict__ nn,
@@ -350,6 +373,8 @@ __global__ __launch_bounds__(64) void k_expand(DevParams P, const float4* __rest
   __shared__ uint32_t tile[UBP_EXPAND_CHUNK][64];
   const uint32_t lane = threadIdx.x;
   const uint32_t i = blockIdx.x * 64u + lane;
+  const uint32_t sc_ = (M && i < P0.n) ? uint32_t(S.sid[i]) : 0u;
+  const DevParams& P = M ? S.sp[sc_] : P0;
   uint32_t mk[27], bk[27];
   uint32_t cnt = 0;
 #pragma unroll
@@ -369,7 +394,7 @@ __global__ __launch_bounds__(64) void k_expand(DevParams P, const float4* __rest
 #pragma unroll
       for (int k = 0; k < 27; ++k) {
         const int dz = k / 9, dy = (k / 3) % 3, dx = k % 3;
-        const uint32_t code = N.sx[dx] | N.sy[dy] | N.sz[dz];
+        const uint32_t code = P.code_base + (N.sx[dx] | N.sy[dy] | N.sz[dz]);
         bk[k] = (mk[k] != 0u) ? start[code] : 0u;  // masks are 0 off the grid
       }
     }
@@ -408,7 +433,8 @@ __global__ __launch_bounds__(64) void k_expand(DevParams P, const float4* __rest
 // Density summation + delta-SPH (old density) + SDF wall volume + Tait EOS.
 // sph.rs:112-160 (density), sph.rs:169-216 (diffusion), lib.rs:232-237,
 // eos.rs:39-47.
-__global__ UBP_LB void k_density(DevParams P, const float4* __restrict__ dr,
+template <bool M>
+__global__ UBP_LB void k_density(DevParams P0, Scenes S, const float4* __restrict__ dr,
                                  const float4* __restrict__ vel, const float4* __restrict__ wall,
                                  const uint8_t* __restrict__ ft, const uint32_t* __restrict__ start,
                                  const uint32_t* __restrict__ nbr, const uint32_t* __restrict__ nn,
@@ -416,8 +442,9 @@ __global__ UBP_LB void k_density(DevParams P, const float4* __restrict__ dr,
                                  float* __restrict__ prs, float4* __restrict__ fr) {
   uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   uint32_t i = blk * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
-  const float dt = dtp[0];
+  if (i >= P0.n) return;
+  UBP_SCENE(i);
+  const float dt = dtp[sc_];
   const float4 pi = dr[2u * i];
   const float rho_i = dr[2u * i + 1u].x;
   Kahan rho, dif;
@@ -474,13 +501,15 @@ __global__ UBP_LB void k_density(DevParams P, const float4* __restrict__ dr,
 
 // Pressure (sph.rs:322-417) + Monaghan viscosity (sph.rs:432-492) + gravity
 // (sph.rs:502-508) + SDF wall pressure and repulsion.
-__global__ UBP_LB void k_forces(DevParams P, const float4* __restrict__ fr,
+template <bool M>
+__global__ UBP_LB void k_forces(DevParams P0, Scenes S, const float4* __restrict__ fr,
                                 const float4* __restrict__ wall, const uint8_t* __restrict__ ft,
                                 const uint32_t* __restrict__ start, const uint32_t* __restrict__ nbr,
                                 const uint32_t* __restrict__ nn, float4* __restrict__ acc) {
   uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   uint32_t i = blk * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
+  if (i >= P0.n) return;
+  UBP_SCENE(i);
   const float4 pi = fr[4u * i];
   const float4 vi = fr[4u * i + 1u];
   const float rho_i = vi.w;
@@ -668,12 +697,14 @@ __global__ UBP_LB void k_adami(DevParams P, const float4* __restrict__ pos,
 
 // v += a dt/2 (lib.rs:301-305); also writes the XSPH record
 // (x y z m)(vx vy vz rho) with the kicked velocity.
-__global__ UBP_LB void k_kick(DevParams P, float4* __restrict__ vel, const float4* __restrict__ acc,
+template <bool M>
+__global__ UBP_LB void k_kick(DevParams P0, Scenes S, float4* __restrict__ vel, const float4* __restrict__ acc,
                               const float4* __restrict__ pos, const float* __restrict__ rho_n,
                               const float* __restrict__ dtp, float4* __restrict__ xr) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
-  const float half_dt = 0.5f * dtp[0];
+  if (i >= P0.n) return;
+  const uint32_t sc_ = M ? uint32_t(S.sid[i]) : 0u;
+  const float half_dt = 0.5f * dtp[sc_];
   float4 v = vel[i];
   float4 a = acc[i];
   v.x += a.x * half_dt;
@@ -688,14 +719,17 @@ __global__ UBP_LB void k_kick(DevParams P, float4* __restrict__ vel, const float
 // compute_timestep (sph.rs:591-628) for the next step: per workgroup
 // max(|v| + c_s) and max |a| as float bits (non-negative, so integer max ==
 // float max) into part[2 * block + {0, 1}]; k_cfl_finalize turns them into dt.
-__global__ UBP_LB void k_kick2_cfl(DevParams P, float4* __restrict__ vel, const float4* __restrict__ acc,
-                                   const float* __restrict__ dtp, uint32_t* __restrict__ part, int do_kick,
-                                   int do_cfl) {
+template <bool M>
+__global__ UBP_LB void k_kick2_cfl(DevParams P0, Scenes S, float4* __restrict__ vel, const float4* __restrict__ acc,
+                                   const float* __restrict__ dtp, uint32_t* __restrict__ part,
+                                   uint32_t* __restrict__ sacc, int do_kick, int do_cfl) {
   __shared__ uint32_t smax[2][4];
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const float half_dt = 0.5f * dtp[0];
+  const uint32_t sc_ = (M && i < P0.n) ? uint32_t(S.sid[i]) : 0u;
+  const DevParams& P = M ? S.sp[sc_] : P0;
+  const float half_dt = 0.5f * dtp[sc_];
   float sig = 0.0f, am = 0.0f;
-  if (i < P.n) {
+  if (i < P0.n) {
     float4 v = vel[i];
     float4 a = acc[i];
     if (do_kick) {
@@ -712,6 +746,66 @@ __global__ UBP_LB void k_kick2_cfl(DevParams P, float4* __restrict__ vel, const 
   }
   if (!do_cfl) return;
   uint32_t bs = __float_as_uint(sig), ba = __float_as_uint(am);
+  if (M) {
+    // Scenes are contiguous slot ranges, so almost every workgroup holds one
+    // scene: it stores (max signal, max |a|, scene) as its partial without
+    // atomics.  A workgroup that straddles scenes marks its partial and adds
+    // into the per-scene maxima sacc[2 s], sacc[2 s + 1] with one atomic per
+    // distinct scene per wave.  Non-negative floats order like their bit
+    // patterns, so integer maxima are exact and order-free.
+    __shared__ uint32_t ssc[4];
+    __shared__ int mixed;
+    const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    const bool valid = i < P0.n;
+    if (threadIdx.x == 0) mixed = 0;
+    const uint32_t first = __shfl(sc_, 0, 64);
+    const bool wave_mixed = __ballot(!valid || sc_ != first) != 0ull;
+    if (lane == 0) ssc[wv] = first;
+    __syncthreads();
+    if (lane == 0 && (wave_mixed || ssc[wv] != ssc[0])) atomicOr(&mixed, 1);
+    __syncthreads();
+    if (!mixed) {
+      for (int o = 32; o > 0; o >>= 1) {
+        uint32_t t0 = __shfl_xor(bs, o, 64);
+        uint32_t t1 = __shfl_xor(ba, o, 64);
+        bs = bs > t0 ? bs : t0;
+        ba = ba > t1 ? ba : t1;
+      }
+      if (lane == 0) { smax[0][wv] = bs; smax[1][wv] = ba; }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        uint32_t m0 = 0, m1 = 0;
+        for (int k = 0; k < int(blockDim.x >> 6); ++k) {
+          m0 = m0 > smax[0][k] ? m0 : smax[0][k];
+          m1 = m1 > smax[1][k] ? m1 : smax[1][k];
+        }
+        part[3 * blockIdx.x] = m0;
+        part[3 * blockIdx.x + 1] = m1;
+        part[3 * blockIdx.x + 2] = ssc[0];
+      }
+    } else {
+      if (threadIdx.x == 0) part[3 * blockIdx.x + 2] = 0xffffffffu;  // no single scene
+      uint64_t rem = __ballot(valid);
+      while (rem) {
+        const int leader = __ffsll((unsigned long long)rem) - 1;
+        const uint32_t lc = __shfl(sc_, leader, 64);
+        const bool mine = valid && sc_ == lc;
+        uint32_t v0 = mine ? bs : 0u, v1 = mine ? ba : 0u;
+        for (int o = 32; o > 0; o >>= 1) {
+          uint32_t t0 = __shfl_xor(v0, o, 64);
+          uint32_t t1 = __shfl_xor(v1, o, 64);
+          v0 = v0 > t0 ? v0 : t0;
+          v1 = v1 > t1 ? v1 : t1;
+        }
+        if (int(lane) == leader) {
+          atomicMax(&sacc[2 * lc], v0);
+          atomicMax(&sacc[2 * lc + 1], v1);
+        }
+        rem &= ~__ballot(mine);
+      }
+    }
+    return;
+  }
   for (int o = 32; o > 0; o >>= 1) {
     uint32_t t0 = __shfl_xor(bs, o, 64);
     uint32_t t1 = __shfl_xor(ba, o, 64);
@@ -771,14 +865,66 @@ __global__ UBP_LB void k_cfl_finalize(DevParams P, const uint32_t* __restrict__ 
   }
 }
 
+// Per-scene dt (many-scene mode): the 64-thread workgroup s reduces the
+// k_kick2_cfl partials of the workgroups over scene s's slot range [lo, hi)
+// (lo = start[code_base] after binning, the input offset before it) with the
+// straddling workgroups' maxima in sacc, which it clears for the next step.
+__global__ __launch_bounds__(64) void k_cfl_finalize_scenes(const DevParams* __restrict__ sp,
+                                                            const uint32_t* __restrict__ part,
+                                                            uint32_t* __restrict__ sacc,
+                                                            const uint32_t* __restrict__ start,
+                                                            const uint32_t* __restrict__ soff, int sorted,
+                                                            uint32_t wg, uint32_t nscenes,
+                                                            float* __restrict__ dts) {
+  const uint32_t sc = blockIdx.x;
+  if (sc >= nscenes) return;
+  const DevParams& P = sp[sc];
+  const uint32_t lo = sorted ? start[P.code_base] : soff[sc];
+  const uint32_t hi = sorted ? start[P.code_base + P.ncodes] : soff[sc + 1u];
+  uint32_t m0 = 0u, m1 = 0u;
+  if (hi > lo) {
+    for (uint32_t b = lo / wg + threadIdx.x; b <= (hi - 1u) / wg; b += 64u) {
+      if (part[3 * b + 2] == sc) {
+        const uint32_t a0 = part[3 * b], a1 = part[3 * b + 1];
+        m0 = m0 > a0 ? m0 : a0;
+        m1 = m1 > a1 ? m1 : a1;
+      }
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t t0 = __shfl_xor(m0, o, 64);
+    uint32_t t1 = __shfl_xor(m1, o, 64);
+    m0 = m0 > t0 ? m0 : t0;
+    m1 = m1 > t1 ? m1 : t1;
+  }
+  if (threadIdx.x == 0) {
+    const uint32_t s0 = sacc[2 * sc], s1 = sacc[2 * sc + 1];
+    m0 = m0 > s0 ? m0 : s0;
+    m1 = m1 > s1 ? m1 : s1;
+    sacc[2 * sc] = 0u;
+    sacc[2 * sc + 1] = 0u;
+    float max_signal = __uint_as_float(m0);
+    if (!(max_signal > P.cs)) max_signal = P.cs;
+    float max_acc = __uint_as_float(m1);
+    float dt_cfl = P.cfl * P.h / max_signal;
+    float dt_force = (max_acc > 1.0e-12f) ? 0.25f * sqrtf(P.h / max_acc) : 0.01f;
+    float dt = (dt_force < dt_cfl) ? dt_force : dt_cfl;
+    if (dt < 1.0e-8f) dt = 1.0e-8f;
+    if (dt > 0.01f) dt = 0.01f;
+    dts[sc] = dt;
+  }
+}
+
 // XSPH correction (sph.rs:236-280) on the post-kick velocities, over the
 // neighbour masks of the last density pass (same positions).
-__global__ UBP_LB void k_xsph(DevParams P, const float4* __restrict__ xr,
+template <bool M>
+__global__ UBP_LB void k_xsph(DevParams P0, Scenes S, const float4* __restrict__ xr,
                               const uint32_t* __restrict__ start, const uint32_t* __restrict__ nbr,
                               const uint32_t* __restrict__ nn, float4* __restrict__ dv) {
   uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   uint32_t i = blk * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
+  if (i >= P0.n) return;
+  UBP_SCENE(i);
   const float4 pi = xr[2u * i];
   const float4 vi = xr[2u * i + 1u];
   const float rho_i = vi.w;
@@ -803,15 +949,17 @@ __global__ UBP_LB void k_xsph(DevParams P, const float4* __restrict__ xr,
 // Drift (lib.rs:313-317), SDF wall projection, domain clamp (boundary.rs:
 // 239-291), then the wall record, cell key and histogram of the new position
 // (neighbor.rs:89-101).
-__global__ UBP_LB void k_drift(DevParams P, const float4* __restrict__ pos,
+template <bool M>
+__global__ UBP_LB void k_drift(DevParams P0, Scenes S, const float4* __restrict__ pos,
                                const float4* __restrict__ vel, const float4* __restrict__ dv,
                                const float* __restrict__ dtp, float4* __restrict__ pos_t,
                                float4* __restrict__ vel_t, float4* __restrict__ wall_t,
                                uint32_t* __restrict__ key, uint32_t* __restrict__ arr,
                                uint32_t* __restrict__ cnt) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
-  const float dt = dtp[0];
+  if (i >= P0.n) return;
+  UBP_SCENE(i);
+  const float dt = dtp[sc_];
   const float4 pi = pos[i];
   const float4 vi = vel[i];
   const float4 d = dv[i];
@@ -853,7 +1001,7 @@ __global__ UBP_LB void k_drift(DevParams P, const float4* __restrict__ pos,
   wall_t[i] = w;
   uint32_t ccx, ccy, ccz;
   pos_to_cell(P, x, y, z, ccx, ccy, ccz);
-  uint32_t code = morton(P, ccx, ccy, ccz);
+  uint32_t code = P.code_base + morton(P, ccx, ccy, ccz);
   key[i] = code;
   arr[i] = wave_histogram_add(cnt, code);
 }
@@ -867,28 +1015,30 @@ __global__ UBP_LB void k_calib_copy(const float4* __restrict__ src, float4* __re
 }
 
 // Cell key of every slot's current position, scattered to original order.
-__global__ UBP_LB void k_debug_keys(DevParams P, const float4* __restrict__ pos,
+template <bool M>
+__global__ UBP_LB void k_debug_keys(DevParams P0, Scenes S, const float4* __restrict__ pos,
                                     const uint32_t* __restrict__ orig, uint32_t* __restrict__ out) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= P.n) return;
+  if (i >= P0.n) return;
+  UBP_SCENE(i);
   float4 p = pos[i];
   uint32_t cx, cy, cz;
   pos_to_cell(P, p.x, p.y, p.z, cx, cy, cz);
-  out[orig[i]] = morton(P, cx, cy, cz);
+  out[orig[i]] = P.code_base + morton(P, cx, cy, cz);
 }
 
 // ErrorMetrics partials (lib.rs:341-377): per block max |rho-rho0|/rho0 (f32)
 // and FP64 sums of energy and mass in a fixed tree order.
-__global__ UBP_LB void k_metrics(DevParams P, const float4* __restrict__ pos,
+__global__ UBP_LB void k_metrics(DevParams P, uint32_t lo, uint32_t hi, const float4* __restrict__ pos,
                                  const float4* __restrict__ vel, const float* __restrict__ rho,
                                  const uint8_t* __restrict__ ft, float* __restrict__ pmax,
                                  double* __restrict__ psum) {
   __shared__ double se[256], sm[256];
   __shared__ float sv[256];
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t i = lo + blockIdx.x * blockDim.x + threadIdx.x;
   double e = 0.0, m = 0.0;
   float var = 0.0f;
-  if (i < P.n) {
+  if (i < hi) {
     float4 p = pos[i];
     float4 v = vel[i];
     int f = ft[i];
