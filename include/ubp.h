@@ -230,6 +230,48 @@ int ubp_get_tuning(const ubp_engine *e, ubp_tuning *out);
 /* Workgroup size per pass (block[UBP_NPASS]); nbr_capacity 0 keeps it. */
 int ubp_set_tuning_ex(ubp_engine *e, const ubp_tuning *t);
 
+/* ---- Learned surrogate (no reference counterpart).
+ * A per-particle network predicts the acceleration from features pooled over
+ * the radius-2h neighbour lists (kernel-weighted sums of relative positions,
+ * velocities, density and last-corrected accelerations; SDF wall distance):
+ *   y = W4 relu(W3 relu(W2 relu(W1 x + b1) + b2) + b3) + b4,
+ *   x = bf16((f - in_mean) * in_inv_std),   a = a_last + |g| (out_mean + out_std y).
+ * Weights are rounded to BF16 for the MFMA products (FP32 accumulate);
+ * activations are rounded to BF16 between layers.  Host structs hold FP32. */
+#define UBP_MLP_IN 32
+#define UBP_MLP_HID 128
+#define UBP_MLP_OUT 3
+typedef struct {
+  float w1[UBP_MLP_IN * UBP_MLP_HID];   /* [in][hid], row-major */
+  float b1[UBP_MLP_HID];
+  float w2[UBP_MLP_HID * UBP_MLP_HID];
+  float b2[UBP_MLP_HID];
+  float w3[UBP_MLP_HID * UBP_MLP_HID];
+  float b3[UBP_MLP_HID];
+  float w4[UBP_MLP_HID * UBP_MLP_OUT];
+  float b4[UBP_MLP_OUT];
+  /* normalisation (not trained) */
+  float in_mean[UBP_MLP_IN], in_inv_std[UBP_MLP_IN];
+  float out_mean[UBP_MLP_OUT], out_std[UBP_MLP_OUT];
+} ubp_mlp_model;
+/* Number of trained parameters (w1 .. b4, contiguous at the start). */
+#define UBP_MLP_NPARAM (UBP_MLP_IN * UBP_MLP_HID + UBP_MLP_HID + 2 * (UBP_MLP_HID * UBP_MLP_HID + UBP_MLP_HID) + UBP_MLP_HID * UBP_MLP_OUT + UBP_MLP_OUT)
+
+/* Network output y (normalised, rows x 3) for raw features x (rows x 32),
+ * host buffers, through the engine's BF16-MFMA inference kernel on the
+ * current device.  Synchronous. */
+int ubp_mlp_forward(const ubp_mlp_model *m, const float *x, size_t rows, float *y);
+/* Loss L = sum over rows and outputs of (y - t)^2 / (3 rows) for normalised
+ * targets t (rows x 3), and dL/dparams through the training kernels
+ * (fused BF16-MFMA forward + backward per 128-row tile, deterministic
+ * reduction).  grad uses the parameter fields of ubp_mlp_model.  Synchronous. */
+int ubp_mlp_loss_grad(const ubp_mlp_model *m, const float *x, const float *t, size_t rows,
+                      ubp_mlp_model *grad, double *loss);
+/* One 16x16x32 BF16 MFMA on fixed inputs against a host product: writes the
+ * largest absolute difference (0 when the lane layout the kernels assume
+ * holds).  Diagnostic, synchronous. */
+int ubp_mfma_selftest(double *max_abs_err);
+
 #ifdef __cplusplus
 }
 #endif

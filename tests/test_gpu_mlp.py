@@ -1,0 +1,81 @@
+"""Surrogate network kernels (BF16 MFMA, FP32 accumulate) against the CPU
+oracle with the same rounding points.  The device accumulates in FP32 in its
+own order while the oracle accumulates in FP64, so a BF16 rounding of an
+activation can land on the other side of a rounding boundary; tolerances are
+one BF16 ulp of each output's magnitude (forward) and 0.1% relative norm per
+parameter block (gradients)."""
+import sys
+
+import numpy as np
+import pytest
+
+import mlp_oracle as mo
+import ultraballpit_amd as ubp
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not ubp.gpu_available():
+        pytest.fail("no gfx950 device / engine library not loadable")
+
+
+def case(seed, rows):
+    rng = np.random.default_rng(seed)
+    m = mo.init_model(seed)
+    for k in ("b1", "b2", "b3", "b4"):
+        m[k] = rng.normal(0, 0.1, m[k].shape).astype(np.float32)
+    m["in_mean"] = rng.normal(0, 0.5, mo.IN).astype(np.float32)
+    m["in_inv_std"] = rng.uniform(0.5, 2.0, mo.IN).astype(np.float32)
+    x = rng.normal(0, 1, (rows, mo.IN)).astype(np.float32)
+    t = rng.normal(0, 1, (rows, mo.OUT)).astype(np.float32)
+    return m, x, t
+
+
+def test_mfma_lane_layout():
+    say("case: one 16x16x32 BF16 MFMA vs host product")
+    assert ubp.mfma_selftest() == 0.0
+
+
+@pytest.mark.parametrize("rows", [1, 63, 64, 65, 1000, 20000])
+def test_forward_matches_oracle(rows):
+    say("case: mlp forward rows", rows)
+    m, x, _ = case(rows, rows)
+    got = ubp.mlp_forward(m, x)
+    want, acts = mo.forward(m, x, keep=True)
+    mag = np.abs(acts[3]) @ np.abs(mo.bf16(m["w4"]).astype(np.float64)) + np.abs(m["b4"])
+    err = np.abs(got - want)
+    assert np.all(err <= 2 ** -7 * mag + 1e-6), float(np.max(err / (mag + 1e-6)))
+    exact = np.mean(err <= 1e-5 * (mag + 1e-6))
+    say(f"  max err / magnitude {np.max(err / (mag + 1e-6)):.2e}, within 1e-5: {exact:.4f}")
+    assert exact > 0.95
+
+
+@pytest.mark.parametrize("rows", [128, 300, 4096])
+def test_loss_and_gradients_match_oracle(rows):
+    say("case: mlp loss/grad rows", rows)
+    m, x, t = case(100 + rows, rows)
+    loss, g = ubp.mlp_loss_grad(m, x, t)
+    L, go = mo.loss_grad(m, x, t)
+    assert loss == pytest.approx(L, rel=1e-4)
+    for k, _ in mo.PARAMS:
+        rel = np.linalg.norm(g[k] - go[k]) / (np.linalg.norm(go[k]) + 1e-30)
+        say(f"  {k}: relative error {rel:.2e}")
+        assert rel < 1e-3, k
+
+
+def test_gradient_descends_the_oracle_loss():
+    # a small step along the device gradient lowers the FP64 oracle loss
+    m, x, t = case(7, 512)
+    _, g = ubp.mlp_loss_grad(m, x, t)
+    m64 = {k: np.asarray(v, np.float64) for k, v in m.items()}
+    L0 = mo.loss_grad(m64, x, t, rounding=False)[0]
+    step = {k: (m64[k] - 1e-2 * g[k]) if k in g else m64[k] for k in m64}
+    L1 = mo.loss_grad(step, x, t, rounding=False)[0]
+    assert L1 < L0

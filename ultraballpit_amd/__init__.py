@@ -87,6 +87,31 @@ class _Tuning(C.Structure):
     _fields_ = [("block", C.c_uint32 * NPASS), ("nbr_capacity", C.c_uint32), ("step_ms", C.c_double)]
 
 
+MLP_IN, MLP_HID, MLP_OUT = 32, 128, 3
+MLP_FIELDS = (("w1", (MLP_IN, MLP_HID)), ("b1", (MLP_HID,)), ("w2", (MLP_HID, MLP_HID)), ("b2", (MLP_HID,)),
+              ("w3", (MLP_HID, MLP_HID)), ("b3", (MLP_HID,)), ("w4", (MLP_HID, MLP_OUT)), ("b4", (MLP_OUT,)),
+              ("in_mean", (MLP_IN,)), ("in_inv_std", (MLP_IN,)), ("out_mean", (MLP_OUT,)),
+              ("out_std", (MLP_OUT,)))
+
+
+class _MlpModel(C.Structure):
+    _fields_ = [(k, C.c_float * int(np.prod(s))) for k, s in MLP_FIELDS]
+
+
+def _model_to_c(m: dict) -> _MlpModel:
+    c = _MlpModel()
+    for k, s in MLP_FIELDS:
+        a = np.ascontiguousarray(m[k], np.float32).ravel()
+        if a.size != int(np.prod(s)):
+            raise ValueError(f"model field {k}: {a.size} values, expected {int(np.prod(s))}")
+        C.memmove(getattr(c, k), a.ctypes.data, a.nbytes)
+    return c
+
+
+def _model_from_c(c: _MlpModel) -> dict:
+    return {k: np.ctypeslib.as_array(getattr(c, k)).astype(np.float32).reshape(s).copy() for k, s in MLP_FIELDS}
+
+
 class _Scene(C.Structure):
     _fields_ = [("particles", C.POINTER(_ParticlesIn)), ("h", C.c_float), ("gravity", C.c_float * 3),
                 ("speed_of_sound", C.c_float), ("cfl_number", C.c_float),
@@ -105,7 +130,8 @@ EXPORTS = ("ubp_create", "ubp_destroy", "ubp_last_error", "ubp_gpu_available",
            "ubp_save_checkpoint", "ubp_restore_checkpoint", "ubp_debug_bins", "ubp_grid_info",
            "ubp_step_n_profiled", "ubp_calibration_copy", "ubp_create_batch", "ubp_scene_count",
            "ubp_scene_range", "ubp_get_scene_dts", "ubp_scene_error_metrics", "ubp_autotune",
-           "ubp_get_tuning", "ubp_set_tuning_ex")
+           "ubp_get_tuning", "ubp_set_tuning_ex", "ubp_mlp_forward", "ubp_mlp_loss_grad",
+           "ubp_mfma_selftest")
 
 #: kernel order of ubp_step_n_profiled
 KERNELS = ("k_kick", "k_xsph", "k_drift", "k_scan", "k_scatter", "k_rank_gather", "k_neigh", "k_expand",
@@ -164,6 +190,10 @@ def lib():
     L.ubp_autotune.argtypes = [C.c_void_p, C.c_int, C.POINTER(_Tuning)]
     L.ubp_get_tuning.argtypes = [C.c_void_p, C.POINTER(_Tuning)]
     L.ubp_set_tuning_ex.argtypes = [C.c_void_p, C.POINTER(_Tuning)]
+    L.ubp_mlp_forward.argtypes = [C.POINTER(_MlpModel), fp, C.c_size_t, fp]
+    L.ubp_mlp_loss_grad.argtypes = [C.POINTER(_MlpModel), fp, fp, C.c_size_t, C.POINTER(_MlpModel),
+                                    C.POINTER(C.c_double)]
+    L.ubp_mfma_selftest.argtypes = [C.POINTER(C.c_double)]
     for name in EXPORTS:
         getattr(L, name)
     _lib = L
@@ -491,6 +521,35 @@ class HipBatch(HipKernel):
         m = _Metrics()
         _check(lib().ubp_scene_error_metrics(self._h, int(s), C.byref(m)), "scene_error_metrics")
         return ErrorMetrics(m.max_density_variation, m.energy_conservation, m.mass_conservation)
+
+
+def mlp_forward(model: dict, x) -> np.ndarray:
+    """Surrogate network output y (rows x 3, normalised) on the GPU (BF16 MFMA)."""
+    x = np.ascontiguousarray(x, np.float32)
+    rows = x.shape[0]
+    y = np.zeros((rows, MLP_OUT), np.float32)
+    mc = _model_to_c(model)
+    _check(lib().ubp_mlp_forward(C.byref(mc), _fp(x), rows, _fp(y)), "mlp_forward")
+    return y
+
+
+def mlp_loss_grad(model: dict, x, t):
+    """(loss, gradient dict) of the surrogate network on the GPU training kernels."""
+    x = np.ascontiguousarray(x, np.float32)
+    t = np.ascontiguousarray(t, np.float32)
+    mc = _model_to_c(model)
+    gc = _MlpModel()
+    loss = C.c_double(0.0)
+    _check(lib().ubp_mlp_loss_grad(C.byref(mc), _fp(x), _fp(t), x.shape[0], C.byref(gc), C.byref(loss)),
+           "mlp_loss_grad")
+    g = _model_from_c(gc)
+    return float(loss.value), {k: g[k] for k, _ in MLP_FIELDS[:8]}
+
+
+def mfma_selftest() -> float:
+    e = C.c_double(-1.0)
+    _check(lib().ubp_mfma_selftest(C.byref(e)), "mfma_selftest")
+    return float(e.value)
 
 
 def calibration_copy(nbytes: int, reps: int = 20) -> float:

@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstddef>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -67,6 +68,10 @@ template <class T> static void dfree(T*& p) {
   if (p) (void)hipFree(p);
   p = nullptr;
 }
+
+}  // namespace ubp
+#include "ubp_learn.hip"
+namespace ubp {
 
 // Morton grid of one scene (neighbor.rs:29-46 dims; cell = 2h, lib.rs:149).
 static void grid_params(DevParams& Q, uint32_t bits[3], float h, const float g[3], float cs, float cfl,
@@ -1377,6 +1382,101 @@ int ubp_set_tuning_ex(ubp_engine* e, const ubp_tuning* t) {
   if (!t) throw Error(UBP_EINVAL, "null tuning");
   e->eng.set_pass_blocks(t->block);
   e->eng.set_capacity(t->nbr_capacity);
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_mlp_forward(const ubp_mlp_model* m, const float* x, size_t rows, float* y) {
+  UBP_API_BEGIN
+  if (!m || (rows && (!x || !y))) throw Error(UBP_EINVAL, "null argument");
+  if (rows > (size_t(1) << 30)) throw Error(UBP_EOVERFLOW, "too many rows");
+  ubp::require_gfx950();
+  if (rows == 0) return UBP_OK;
+  ubp::ScopedStream S;
+  ubp::MlpBuf M;
+  M.upload(m, S.s);
+  ubp::DevArr<float> dx(rows * ubp::MLP_IN), dy(rows * ubp::MLP_OUT);
+  HIPCHK(hipMemcpyAsync(dx.p, x, rows * ubp::MLP_IN * 4, hipMemcpyHostToDevice, S.s));
+  hipLaunchKernelGGL(ubp::k_mlp_fwd<0>, dim3(ubp::mlp_fwd_grid(rows)), dim3(256), 0, S.s, M.view(),
+                     (const float*)dx.p, uint32_t(rows), dy.p, (const float4*)nullptr, (float4*)nullptr, 0.0f);
+  LAUNCHCHK("k_mlp_fwd");
+  HIPCHK(hipMemcpyAsync(y, dy.p, rows * ubp::MLP_OUT * 4, hipMemcpyDeviceToHost, S.s));
+  HIPCHK(hipStreamSynchronize(S.s));
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_mlp_loss_grad(const ubp_mlp_model* m, const float* x, const float* t, size_t rows, ubp_mlp_model* grad,
+                      double* loss) {
+  UBP_API_BEGIN
+  if (!m || !grad || (rows && (!x || !t))) throw Error(UBP_EINVAL, "null argument");
+  if (rows == 0 || rows > (size_t(1) << 26)) throw Error(UBP_EINVAL, "rows must be in [1, 2^26]");
+  ubp::require_gfx950();
+  ubp::ScopedStream S;
+  ubp::MlpBuf M;
+  M.upload(m, S.s);
+  const uint32_t nblk = uint32_t((rows + ubp::TR - 1) / ubp::TR);
+  ubp::DevArr<float> dx(rows * ubp::MLP_IN), dt(rows * 4), part(size_t(nblk) * (ubp::MLP_NPARAM + 1)),
+      dg(ubp::MLP_NPARAM + 1);
+  ubp::DevArr<uint32_t> didx(rows);
+  std::vector<float> ht(rows * 4, 0.0f);
+  for (size_t r = 0; r < rows; ++r)
+    for (int c = 0; c < 3; ++c) ht[r * 4 + c] = t[r * 3 + c];
+  std::vector<uint32_t> hi(rows);
+  for (size_t r = 0; r < rows; ++r) hi[r] = uint32_t(r);
+  HIPCHK(hipMemcpyAsync(dx.p, x, rows * ubp::MLP_IN * 4, hipMemcpyHostToDevice, S.s));
+  HIPCHK(hipMemcpyAsync(dt.p, ht.data(), rows * 16, hipMemcpyHostToDevice, S.s));
+  HIPCHK(hipMemcpyAsync(didx.p, hi.data(), rows * 4, hipMemcpyHostToDevice, S.s));
+  ubp::mlp_grad(M, dx.p, dt.p, didx.p, uint32_t(rows), part.p, dg.p, S.s);
+  std::vector<float> hg(ubp::MLP_NPARAM + 1);
+  HIPCHK(hipMemcpyAsync(hg.data(), dg.p, hg.size() * 4, hipMemcpyDeviceToHost, S.s));
+  HIPCHK(hipStreamSynchronize(S.s));
+  std::memset(grad, 0, sizeof(*grad));
+  std::memcpy(grad, hg.data(), ubp::MLP_NPARAM * 4);
+  if (loss) *loss = double(hg[ubp::MLP_NPARAM]) / (3.0 * double(rows));
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_mfma_selftest(double* max_abs_err) {
+  UBP_API_BEGIN
+  if (!max_abs_err) throw Error(UBP_EINVAL, "null argument");
+  ubp::require_gfx950();
+  // small dyadic values: every product and sum is exact in FP32
+  std::vector<uint16_t> A(16 * 32), B(32 * 16);
+  std::vector<double> ha(16 * 32), hb(32 * 16);
+  auto bf = [](float f) { uint32_t u; std::memcpy(&u, &f, 4); return uint16_t(u >> 16); };
+  for (int i = 0; i < 16; ++i)
+    for (int k = 0; k < 32; ++k) {
+      float v = float(((i * 7 + k * 3) % 17) - 8) * 0.125f;
+      A[i * 32 + k] = bf(v);
+      ha[i * 32 + k] = v;
+    }
+  for (int k = 0; k < 32; ++k)
+    for (int j = 0; j < 16; ++j) {
+      float v = float(((k * 5 + j * 11) % 13) - 6) * 0.25f;
+      B[k * 16 + j] = bf(v);
+      hb[k * 16 + j] = v;
+    }
+  ubp::ScopedStream S;
+  ubp::DevArr<uint16_t> da(A.size()), db(B.size());
+  ubp::DevArr<float> dd(256);
+  HIPCHK(hipMemcpyAsync(da.p, A.data(), A.size() * 2, hipMemcpyHostToDevice, S.s));
+  HIPCHK(hipMemcpyAsync(db.p, B.data(), B.size() * 2, hipMemcpyHostToDevice, S.s));
+  hipLaunchKernelGGL(ubp::k_mfma_probe, dim3(1), dim3(64), 0, S.s, (const uint16_t*)da.p, (const uint16_t*)db.p,
+                     dd.p);
+  LAUNCHCHK("k_mfma_probe");
+  std::vector<float> D(256);
+  HIPCHK(hipMemcpyAsync(D.data(), dd.p, 1024, hipMemcpyDeviceToHost, S.s));
+  HIPCHK(hipStreamSynchronize(S.s));
+  double mx = 0.0;
+  for (int i = 0; i < 16; ++i)
+    for (int j = 0; j < 16; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 32; ++k) s += ha[i * 32 + k] * hb[k * 16 + j];
+      mx = std::max(mx, std::fabs(s - double(D[i * 16 + j])));
+    }
+  *max_abs_err = mx;
   return UBP_OK;
   UBP_API_END
 }

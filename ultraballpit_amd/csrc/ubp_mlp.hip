@@ -1,0 +1,521 @@
+// ubp_mlp.hip -- BF16-MFMA kernels of the surrogate network (gfx950).
+//
+// Network: 32 -> 128 -> 128 -> 128 -> 3, ReLU, BF16 operands, FP32
+// accumulation (v_mfma_f32_16x16x32_bf16).  Weights live in device memory as
+// pre-packed MFMA B-operand fragments, so every lane loads its operand as one
+// 16-byte vector:
+//   fragment (ntile, kchunk) of a K x N matrix B is 64 lanes x 8 bf16;
+//   lane l holds B[32 kchunk + 8 (l / 16) + e][16 ntile + l % 16], e = 0..7.
+// A operands come from LDS: lane l holds A[16 rt + l % 16][32 q + 8 (l / 16) + e].
+// The 16 x 16 FP32 result: lane l holds D[4 (l / 16) + v][l % 16], v = 0..3.
+// Activations are rounded to BF16 (round to nearest even) between layers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ubp {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int MLP_IN = 32, MLP_HID = 128, MLP_OUT = 3;
+// parameter vector offsets (the field order of ubp_mlp_model)
+constexpr int OFF_W1 = 0;
+constexpr int OFF_B1 = OFF_W1 + MLP_IN * MLP_HID;
+constexpr int OFF_W2 = OFF_B1 + MLP_HID;
+constexpr int OFF_B2 = OFF_W2 + MLP_HID * MLP_HID;
+constexpr int OFF_W3 = OFF_B2 + MLP_HID;
+constexpr int OFF_B3 = OFF_W3 + MLP_HID * MLP_HID;
+constexpr int OFF_W4 = OFF_B3 + MLP_HID;
+constexpr int OFF_B4 = OFF_W4 + MLP_HID * MLP_OUT;
+constexpr int MLP_NPARAM = OFF_B4 + MLP_OUT;
+// packed fragment regions (uint16 elements); 512 per fragment
+constexpr int FR_W1 = 0;                       // W1: K 32, N 128: 8 x 1
+constexpr int FR_W2 = FR_W1 + 8 * 1 * 512;     // W2: 8 x 4
+constexpr int FR_W3 = FR_W2 + 8 * 4 * 512;     // W3: 8 x 4
+constexpr int FR_W4 = FR_W3 + 8 * 4 * 512;     // W4: K 128, N 16 (3 used): 1 x 4
+constexpr int FR_T2 = FR_W4 + 1 * 4 * 512;     // W2^T: 8 x 4
+constexpr int FR_T3 = FR_T2 + 8 * 4 * 512;     // W3^T: 8 x 4
+constexpr int FR_T4 = FR_T3 + 8 * 4 * 512;     // W4^T: K 32 (3 used), N 128: 8 x 1
+constexpr int FR_TOTAL = FR_T4 + 8 * 1 * 512;
+
+// Device view of a model: packed BF16 fragments + FP32 biases and
+// normalisation (pointers into device buffers).
+struct MlpDev {
+  const uint16_t* frag;
+  const float* prm;       // FP32 parameters (biases read from here)
+  const float* in_mean;   // [32]
+  const float* in_istd;   // [32]
+  const float* out_mean;  // [3]
+  const float* out_std;   // [3]
+};
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even (finite inputs)
+  return uint16_t(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(uint32_t(b) << 16); }
+// ReLU derivative from a stored activation: 1 where the BF16 value is > 0
+__device__ __forceinline__ bool bf_pos(uint16_t b) { return int16_t(b) > 0; }
+
+__device__ __forceinline__ bf16x8 ld_frag(const uint16_t* p) {
+  return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(p));
+}
+// 8 values strided by `stride` elements (a column slice of a row-major tile)
+__device__ __forceinline__ bf16x8 ld_col8(const uint16_t* p, int stride) {
+  u32x4 v;
+  v.x = uint32_t(p[0]) | (uint32_t(p[stride]) << 16);
+  v.y = uint32_t(p[2 * stride]) | (uint32_t(p[3 * stride]) << 16);
+  v.z = uint32_t(p[4 * stride]) | (uint32_t(p[5 * stride]) << 16);
+  v.w = uint32_t(p[6 * stride]) | (uint32_t(p[7 * stride]) << 16);
+  return __builtin_bit_cast(bf16x8, v);
+}
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// Position of B[k][n] in a packed region with `nkch` k-chunks per n-tile.
+__device__ __forceinline__ int frag_pos(int k, int n, int nkch) {
+  const int lane = (n & 15) + 16 * ((k & 31) >> 3);
+  return ((n >> 4) * nkch + (k >> 5)) * 512 + lane * 8 + (k & 7);
+}
+
+// Writes the BF16 fragments of FP32 parameter p (forward operand and, for
+// W2..W4, the transposed operand of the backward pass).  Padding entries are
+// never written (the region is zeroed once).
+__device__ __forceinline__ void pack_param(uint16_t* __restrict__ frag, int p, float w) {
+  const uint16_t b = f2bf(w);
+  if (p < OFF_B1) {
+    const int k = p / MLP_HID, n = p % MLP_HID;
+    frag[FR_W1 + frag_pos(k, n, 1)] = b;
+  } else if (p >= OFF_W2 && p < OFF_B2) {
+    const int q = p - OFF_W2, k = q / MLP_HID, n = q % MLP_HID;
+    frag[FR_W2 + frag_pos(k, n, 4)] = b;
+    frag[FR_T2 + frag_pos(n, k, 4)] = b;
+  } else if (p >= OFF_W3 && p < OFF_B3) {
+    const int q = p - OFF_W3, k = q / MLP_HID, n = q % MLP_HID;
+    frag[FR_W3 + frag_pos(k, n, 4)] = b;
+    frag[FR_T3 + frag_pos(n, k, 4)] = b;
+  } else if (p >= OFF_W4 && p < OFF_B4) {
+    const int q = p - OFF_W4, k = q / MLP_OUT, n = q % MLP_OUT;
+    frag[FR_W4 + frag_pos(k, n, 4)] = b;
+    frag[FR_T4 + frag_pos(n, k, 1)] = b;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mlp_pack(const float* __restrict__ prm, uint16_t* __restrict__ frag) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < MLP_NPARAM) pack_param(frag, p, prm[p]);
+}
+
+// LDS row strides (elements): 16-byte aligned rows whose 16 consecutive rows
+// fall in distinct bank groups.
+constexpr int XS = 40;   // 32 features (+ zero pad)
+constexpr int HS = 136;  // 128 hidden
+
+// Inference.  Persistent workgroups of 4 waves walk 64-row tiles.  Wave w
+// computes hidden columns [32 w, 32 w + 32) of every layer with its weight
+// fragments held in registers, and row tile w of the output layer.
+//   MODE 0: Y[row][3] = y (normalised network output);
+//   MODE 1: acc[row] = a_last + |g| (out_mean + out_std y), a_last = aux.xyz.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_mlp_fwd(MlpDev W, const float* __restrict__ X, uint32_t rows,
+                                                 float* __restrict__ Y, const float4* __restrict__ aux,
+                                                 float4* __restrict__ acc, float gmag) {
+  __shared__ __attribute__((aligned(16))) uint16_t Xs[64 * XS];
+  __shared__ __attribute__((aligned(16))) uint16_t Ha[64 * HS];
+  __shared__ __attribute__((aligned(16))) uint16_t Hb[64 * HS];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int l16 = lane & 15, lg = lane >> 4;
+  bf16x8 f1[2], f2[2][4], f3[2][4], f4[4];
+  float bb1[2], bb2[2], bb3[2];
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    const int nt = 2 * w + c;
+    f1[c] = ld_frag(W.frag + FR_W1 + nt * 512 + lane * 8);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      f2[c][q] = ld_frag(W.frag + FR_W2 + (nt * 4 + q) * 512 + lane * 8);
+      f3[c][q] = ld_frag(W.frag + FR_W3 + (nt * 4 + q) * 512 + lane * 8);
+    }
+    bb1[c] = W.prm[OFF_B1 + 16 * nt + l16];
+    bb2[c] = W.prm[OFF_B2 + 16 * nt + l16];
+    bb3[c] = W.prm[OFF_B3 + 16 * nt + l16];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) f4[q] = ld_frag(W.frag + FR_W4 + q * 512 + lane * 8);
+  const float bb4 = l16 < MLP_OUT ? W.prm[OFF_B4 + l16] : 0.0f;
+  const int fc = (tid & 3) * 8;
+  float mu[8], is[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) { mu[e] = W.in_mean[fc + e]; is[e] = W.in_istd[fc + e]; }
+  float om[3], os[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) { om[c] = W.out_mean[c]; os[c] = W.out_std[c]; }
+
+  for (uint32_t base = blockIdx.x * 64u; base < rows; base += gridDim.x * 64u) {
+    {  // stage the normalised BF16 input tile
+      const int r = tid >> 2;
+      const uint32_t row = base + uint32_t(r);
+      uint32_t pk[4] = {0u, 0u, 0u, 0u};
+      if (row < rows) {
+        const float4 a = *reinterpret_cast<const float4*>(X + size_t(row) * MLP_IN + fc);
+        const float4 b = *reinterpret_cast<const float4*>(X + size_t(row) * MLP_IN + fc + 4);
+        const float v[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int e = 0; e < 8; e += 2)
+          pk[e / 2] = uint32_t(f2bf((v[e] - mu[e]) * is[e])) | (uint32_t(f2bf((v[e + 1] - mu[e + 1]) * is[e + 1])) << 16);
+      }
+      *reinterpret_cast<u32x4*>(Xs + r * XS + fc) = u32x4{pk[0], pk[1], pk[2], pk[3]};
+    }
+    __syncthreads();
+    // layer 1: Xs -> Ha
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+      const bf16x8 a = ld_frag(Xs + (16 * rt + l16) * XS + 8 * lg);
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        const f32x4 d = mfma(a, f1[c], f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+        const int col = 16 * (2 * w + c) + l16;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Ha[(16 * rt + 4 * lg + v) * HS + col] = f2bf(fmaxf(d[v] + bb1[c], 0.0f));
+      }
+    }
+    __syncthreads();
+    // layer 2: Ha -> Hb
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+      bf16x8 a[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) a[q] = ld_frag(Ha + (16 * rt + l16) * HS + 32 * q + 8 * lg);
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        f32x4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) d = mfma(a[q], f2[c][q], d);
+        const int col = 16 * (2 * w + c) + l16;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Hb[(16 * rt + 4 * lg + v) * HS + col] = f2bf(fmaxf(d[v] + bb2[c], 0.0f));
+      }
+    }
+    __syncthreads();
+    // layer 3: Hb -> Ha
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+      bf16x8 a[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) a[q] = ld_frag(Hb + (16 * rt + l16) * HS + 32 * q + 8 * lg);
+#pragma unroll
+      for (int c = 0; c < 2; ++c) {
+        f32x4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) d = mfma(a[q], f3[c][q], d);
+        const int col = 16 * (2 * w + c) + l16;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Ha[(16 * rt + 4 * lg + v) * HS + col] = f2bf(fmaxf(d[v] + bb3[c], 0.0f));
+      }
+    }
+    __syncthreads();
+    // output layer: wave w takes row tile w
+    {
+      f32x4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) d = mfma(ld_frag(Ha + (16 * w + l16) * HS + 32 * q + 8 * lg), f4[q], d);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const uint32_t row = base + uint32_t(16 * w + 4 * lg + v);
+        const float y = d[v] + bb4;
+        if (MODE == 0) {
+          if (l16 < MLP_OUT && row < rows) Y[size_t(row) * MLP_OUT + l16] = y;
+        } else {
+          // gather the three outputs of this row into lane 16 lg
+          const float yy = __shfl(y, 16 * lg + 1, 64);
+          const float yz = __shfl(y, 16 * lg + 2, 64);
+          if (l16 == 0 && row < rows) {
+            const float4 al = aux[row];
+            float4 o;
+            o.x = al.x + gmag * (om[0] + os[0] * y);
+            o.y = al.y + gmag * (om[1] + os[1] * yy);
+            o.z = al.z + gmag * (om[2] + os[2] * yz);
+            o.w = 0.0f;
+            acc[row] = o;
+          }
+        }
+      }
+    }
+    __syncthreads();  // the next tile overwrites Xs and Ha
+  }
+}
+
+// Training: one 128-row tile per workgroup, fused forward + backward.  Rows
+// are idx[base + r] of the data set (r >= batch rows contribute nothing).
+// The workgroup writes its partial gradient (model layout) and squared-error
+// sum to part[blockIdx.x][MLP_NPARAM + 1].  dY = bf16(2 (y - t) / (3 batch)).
+constexpr int TR = 128;
+__global__ __launch_bounds__(256) void k_mlp_train(MlpDev W, const float* __restrict__ X,
+                                                   const float* __restrict__ T, const uint32_t* __restrict__ idx,
+                                                   uint32_t batch, float* __restrict__ part) {
+  __shared__ __attribute__((aligned(16))) uint16_t Xs[TR * XS];
+  __shared__ __attribute__((aligned(16))) uint16_t H1[TR * HS];
+  __shared__ __attribute__((aligned(16))) uint16_t H2[TR * HS];
+  __shared__ __attribute__((aligned(16))) uint16_t H3[TR * HS];  // later dZ2
+  __shared__ __attribute__((aligned(16))) uint16_t D[TR * HS];   // dZ3, later dZ1
+  __shared__ __attribute__((aligned(16))) uint16_t DY[TR * XS];  // dY, 32 columns (3 used)
+  __shared__ float sloss[4];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int l16 = lane & 15, lg = lane >> 4;
+  const uint32_t base = blockIdx.x * TR;
+  float* out = part + size_t(blockIdx.x) * (MLP_NPARAM + 1);
+  const float scale = 2.0f / (3.0f * float(batch));
+
+  // stage inputs (2 rows per thread-quad pass) and clear dY
+  for (int it = tid; it < TR * 4; it += 256) {
+    const int r = it >> 2, fc = (it & 3) * 8;
+    const uint32_t g = base + uint32_t(r);
+    uint32_t pk[4] = {0u, 0u, 0u, 0u};
+    if (g < batch) {
+      const uint32_t row = idx[g];
+      const float* xr = X + size_t(row) * MLP_IN + fc;
+#pragma unroll
+      for (int e = 0; e < 8; e += 2) {
+        const float v0 = (xr[e] - W.in_mean[fc + e]) * W.in_istd[fc + e];
+        const float v1 = (xr[e + 1] - W.in_mean[fc + e + 1]) * W.in_istd[fc + e + 1];
+        pk[e / 2] = uint32_t(f2bf(v0)) | (uint32_t(f2bf(v1)) << 16);
+      }
+    }
+    *reinterpret_cast<u32x4*>(Xs + r * XS + fc) = u32x4{pk[0], pk[1], pk[2], pk[3]};
+  }
+  for (int it = tid; it < TR * XS / 8; it += 256)
+    *reinterpret_cast<u32x4*>(DY + it * 8) = u32x4{0u, 0u, 0u, 0u};
+  __syncthreads();
+
+  // ---- forward ----
+  {  // layer 1
+    const bf16x8 b0 = ld_frag(W.frag + FR_W1 + (2 * w) * 512 + lane * 8);
+    const bf16x8 b1 = ld_frag(W.frag + FR_W1 + (2 * w + 1) * 512 + lane * 8);
+    const float c0 = W.prm[OFF_B1 + 32 * w + l16], c1 = W.prm[OFF_B1 + 32 * w + 16 + l16];
+    for (int rt = 0; rt < TR / 16; ++rt) {
+      const bf16x8 a = ld_frag(Xs + (16 * rt + l16) * XS + 8 * lg);
+      const f32x4 d0 = mfma(a, b0, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+      const f32x4 d1 = mfma(a, b1, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        H1[(16 * rt + 4 * lg + v) * HS + 32 * w + l16] = f2bf(fmaxf(d0[v] + c0, 0.0f));
+        H1[(16 * rt + 4 * lg + v) * HS + 32 * w + 16 + l16] = f2bf(fmaxf(d1[v] + c1, 0.0f));
+      }
+    }
+  }
+  __syncthreads();
+  auto hidden = [&](const uint16_t* Hin, uint16_t* Hout, int fr, int ob) {
+    bf16x8 b[2][4];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) b[c][q] = ld_frag(W.frag + fr + ((2 * w + c) * 4 + q) * 512 + lane * 8);
+    const float c0 = W.prm[ob + 32 * w + l16], c1 = W.prm[ob + 32 * w + 16 + l16];
+    for (int rt = 0; rt < TR / 16; ++rt) {
+      bf16x8 a[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) a[q] = ld_frag(Hin + (16 * rt + l16) * HS + 32 * q + 8 * lg);
+      f32x4 d0 = {0.0f, 0.0f, 0.0f, 0.0f}, d1 = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) { d0 = mfma(a[q], b[0][q], d0); d1 = mfma(a[q], b[1][q], d1); }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        Hout[(16 * rt + 4 * lg + v) * HS + 32 * w + l16] = f2bf(fmaxf(d0[v] + c0, 0.0f));
+        Hout[(16 * rt + 4 * lg + v) * HS + 32 * w + 16 + l16] = f2bf(fmaxf(d1[v] + c1, 0.0f));
+      }
+    }
+  };
+  hidden(H1, H2, FR_W2, OFF_B2);
+  __syncthreads();
+  hidden(H2, H3, FR_W3, OFF_B3);
+  __syncthreads();
+  {  // output layer, loss and dY: wave w takes row tiles 2w, 2w + 1
+    bf16x8 b[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) b[q] = ld_frag(W.frag + FR_W4 + q * 512 + lane * 8);
+    const float c4 = l16 < MLP_OUT ? W.prm[OFF_B4 + l16] : 0.0f;
+    float ls = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int rt = 2 * w + t;
+      f32x4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) d = mfma(ld_frag(H3 + (16 * rt + l16) * HS + 32 * q + 8 * lg), b[q], d);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = 16 * rt + 4 * lg + v;
+        const uint32_t g = base + uint32_t(r);
+        if (l16 < MLP_OUT && g < batch) {
+          const float e = (d[v] + c4) - T[size_t(idx[g]) * 4 + l16];
+          ls += e * e;
+          DY[r * XS + l16] = f2bf(e * scale);
+        }
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) ls += __shfl_xor(ls, o, 64);
+    if (lane == 0) sloss[w] = ls;
+  }
+  __syncthreads();
+  if (tid == 0) out[MLP_NPARAM] = (sloss[0] + sloss[1]) + (sloss[2] + sloss[3]);
+
+  // ---- backward ----
+  // dW4 = H3^T dY (128 x 3): wave w takes feature tiles 2w, 2w + 1
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int ft = 2 * w + t;
+    f32x4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int q = 0; q < TR / 32; ++q)
+      d = mfma(ld_col8(H3 + (32 * q + 8 * lg) * HS + 16 * ft + l16, HS), ld_col8(DY + (32 * q + 8 * lg) * XS + l16, XS), d);
+    if (l16 < MLP_OUT)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) out[OFF_W4 + (16 * ft + 4 * lg + v) * MLP_OUT + l16] = d[v];
+  }
+  if (tid < MLP_OUT) {
+    float s = 0.0f;
+    for (int r = 0; r < TR; ++r) s += bf2f(DY[r * XS + tid]);
+    out[OFF_B4 + tid] = s;
+  }
+  {  // dZ3 = (dY W4^T) * (H3 > 0) -> D
+    const bf16x8 b0 = ld_frag(W.frag + FR_T4 + (2 * w) * 512 + lane * 8);
+    const bf16x8 b1 = ld_frag(W.frag + FR_T4 + (2 * w + 1) * 512 + lane * 8);
+    for (int rt = 0; rt < TR / 16; ++rt) {
+      const bf16x8 a = ld_frag(DY + (16 * rt + l16) * XS + 8 * lg);
+      const f32x4 d0 = mfma(a, b0, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+      const f32x4 d1 = mfma(a, b1, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = 16 * rt + 4 * lg + v, c0 = 32 * w + l16, c1 = c0 + 16;
+        D[r * HS + c0] = bf_pos(H3[r * HS + c0]) ? f2bf(d0[v]) : uint16_t(0);
+        D[r * HS + c1] = bf_pos(H3[r * HS + c1]) ? f2bf(d1[v]) : uint16_t(0);
+      }
+    }
+  }
+  __syncthreads();
+  // dW = Hin^T dZ (128 x 128) and db = column sums of dZ; wave w: feature tiles 2w, 2w+1
+  auto wgrad = [&](const uint16_t* Hin, const uint16_t* dZ, int ow, int ob) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int ft = 2 * w + t;
+      bf16x8 a[TR / 32];
+#pragma unroll
+      for (int q = 0; q < TR / 32; ++q) a[q] = ld_col8(Hin + (32 * q + 8 * lg) * HS + 16 * ft + l16, HS);
+      for (int ct = 0; ct < MLP_HID / 16; ++ct) {
+        f32x4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int q = 0; q < TR / 32; ++q) d = mfma(a[q], ld_col8(dZ + (32 * q + 8 * lg) * HS + 16 * ct + l16, HS), d);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) out[ow + (16 * ft + 4 * lg + v) * MLP_HID + 16 * ct + l16] = d[v];
+      }
+    }
+    if (tid < MLP_HID) {
+      float s = 0.0f;
+      for (int r = 0; r < TR; ++r) s += bf2f(dZ[r * HS + tid]);
+      out[ob + tid] = s;
+    }
+  };
+  // dZ_prev = (dZ W^T) * (Hprev > 0)
+  auto dgrad = [&](const uint16_t* dZ, const uint16_t* Hprev, uint16_t* dOut, int fr) {
+    bf16x8 b[2][4];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) b[c][q] = ld_frag(W.frag + fr + ((2 * w + c) * 4 + q) * 512 + lane * 8);
+    for (int rt = 0; rt < TR / 16; ++rt) {
+      bf16x8 a[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) a[q] = ld_frag(dZ + (16 * rt + l16) * HS + 32 * q + 8 * lg);
+      f32x4 d0 = {0.0f, 0.0f, 0.0f, 0.0f}, d1 = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) { d0 = mfma(a[q], b[0][q], d0); d1 = mfma(a[q], b[1][q], d1); }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = 16 * rt + 4 * lg + v, c0 = 32 * w + l16, c1 = c0 + 16;
+        dOut[r * HS + c0] = bf_pos(Hprev[r * HS + c0]) ? f2bf(d0[v]) : uint16_t(0);
+        dOut[r * HS + c1] = bf_pos(Hprev[r * HS + c1]) ? f2bf(d1[v]) : uint16_t(0);
+      }
+    }
+  };
+  wgrad(H2, D, OFF_W3, OFF_B3);
+  dgrad(D, H2, H3, FR_T3);  // dZ2 -> H3 (H3 is dead after dZ3)
+  __syncthreads();
+  wgrad(H1, H3, OFF_W2, OFF_B2);
+  dgrad(H3, H1, D, FR_T2);  // dZ1 -> D (dZ3 is dead)
+  __syncthreads();
+  // dW1 = Xs^T dZ1 (32 x 128): wave w takes column tiles 2w, 2w + 1
+#pragma unroll
+  for (int ft = 0; ft < 2; ++ft) {
+    bf16x8 a[TR / 32];
+#pragma unroll
+    for (int q = 0; q < TR / 32; ++q) a[q] = ld_col8(Xs + (32 * q + 8 * lg) * XS + 16 * ft + l16, XS);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int ct = 2 * w + t;
+      f32x4 d = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int q = 0; q < TR / 32; ++q) d = mfma(a[q], ld_col8(D + (32 * q + 8 * lg) * HS + 16 * ct + l16, HS), d);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) out[OFF_W1 + (16 * ft + 4 * lg + v) * MLP_HID + 16 * ct + l16] = d[v];
+    }
+  }
+  if (tid < MLP_HID) {
+    float s = 0.0f;
+    for (int r = 0; r < TR; ++r) s += bf2f(D[r * HS + tid]);
+    out[OFF_B1 + tid] = s;
+  }
+}
+
+// grad[p] = sum over workgroups of part[b][p], in workgroup order.
+__global__ __launch_bounds__(256) void k_mlp_reduce(const float* __restrict__ part, uint32_t nblk,
+                                                    float* __restrict__ grad) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > MLP_NPARAM) return;
+  float s = 0.0f;
+  for (uint32_t b = 0; b < nblk; ++b) s += part[size_t(b) * (MLP_NPARAM + 1) + p];
+  grad[p] = s;
+}
+
+// Adam (FP32 master weights), then the BF16 fragments of the new weights.
+__global__ __launch_bounds__(256) void k_mlp_adam(float* __restrict__ prm, float* __restrict__ m1,
+                                                  float* __restrict__ m2, const float* __restrict__ grad,
+                                                  float lr, float beta1, float beta2, float eps, float c1,
+                                                  float c2, uint16_t* __restrict__ frag) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= MLP_NPARAM) return;
+  const float g = grad[p];
+  const float a = beta1 * m1[p] + (1.0f - beta1) * g;
+  const float b = beta2 * m2[p] + (1.0f - beta2) * g * g;
+  m1[p] = a;
+  m2[p] = b;
+  const float wv = prm[p] - lr * (a / c1) / (sqrtf(b / c2) + eps);
+  prm[p] = wv;
+  pack_param(frag, p, wv);
+}
+
+// batch row indices: idx[i] = hash(seed, step, i) mod rows
+__global__ __launch_bounds__(256) void k_mlp_sample(uint32_t* __restrict__ idx, uint32_t batch, uint32_t rows,
+                                                    uint64_t seed, uint32_t step) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= batch) return;
+  uint64_t z = seed + 0x9e3779b97f4a7c15ull * (uint64_t(step) * 0x100000001b3ull + i + 1u);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  z ^= z >> 31;
+  idx[i] = uint32_t(z % rows);
+}
+
+__global__ void k_mfma_probe(const uint16_t* __restrict__ A, const uint16_t* __restrict__ B, float* __restrict__ D) {
+  // A 16 x 32 and B 32 x 16 row-major bf16; D 16 x 16 row-major
+  const int lane = threadIdx.x, l16 = lane & 15, lg = lane >> 4;
+  const bf16x8 a = ld_frag(A + l16 * 32 + 8 * lg);
+  const bf16x8 b = ld_col8(B + (8 * lg) * 16 + l16, 16);
+  const f32x4 d = mfma(a, b, f32x4{0.0f, 0.0f, 0.0f, 0.0f});
+#pragma unroll
+  for (int v = 0; v < 4; ++v) D[(4 * lg + v) * 16 + l16] = d[v];
+}
+
+}  // namespace ubp
