@@ -129,6 +129,56 @@ def sweep_bench(steps: int, warmup: int, trial: int):
             "dt_min": float(dts.min()), "dt_max": float(dts.max()), "state_finite": finite}
 
 
+def surrogate_bench(cycles: int, warm_steps: int, model_path: str):
+    """Surrogate/plain speedup on the 1M dam break: after `warm_steps` plain
+    steps at a fixed dt (0.9 x the initial CFL dt), `cycles` correction
+    cycles (8 network steps at 4 dt + 1 plain step) against the plain solver
+    over the same simulated time; position error = mean |x_net - x_plain| /
+    mean |x_plain - x_start|.  Also: the same cycles without the network
+    (a = a_last), the plain solver's own dt-vs-dt/2 difference on the same
+    metric (its discretisation floor), and a run judged by plain reruns."""
+    from ultraballpit_amd import surrogate
+    if not os.path.exists(model_path):
+        return {"skipped": f"no trained model at {os.path.relpath(model_path, ROOT)}"}
+    model = ubp.load_model(model_path)
+    sc = scenes.dam_break()
+    dt = surrogate.fixed_dt(sc)
+    net = surrogate.evaluate(sc, model, cycles=cycles, dt=dt, warm_steps=warm_steps)
+    held = surrogate.evaluate(sc, surrogate.zero_model(model), cycles=cycles, dt=dt, warm_steps=warm_steps)
+    judged = surrogate.evaluate(sc, model, cycles=cycles, dt=dt, warm_steps=warm_steps, verify_every=4,
+                                threshold=0.01)
+    # discretisation floor: plain at dt/2 against plain at dt
+    nsteps = net["plain_steps"]
+    pos = []
+    for sub in (1, 2):
+        e = ubp.HipKernel.from_scene(sc)
+        for _ in range(warm_steps):
+            e.step_async(dt)
+        e.sync()
+        p = e.particles()
+        x0 = np.stack([p["x"], p["y"], p["z"]], 1).astype(np.float64)
+        for _ in range(nsteps * sub):
+            e.step_async(dt / sub)
+        e.sync()
+        p = e.particles()
+        pos.append(np.stack([p["x"], p["y"], p["z"]], 1).astype(np.float64))
+        e.close()
+    disp = float(np.mean(np.linalg.norm(pos[0] - x0, axis=1)))
+    floor = float(np.mean(np.linalg.norm(pos[1] - pos[0], axis=1))) / disp if disp > 0 else None
+    info_path = os.path.splitext(model_path)[0] + ".json"
+    train_info = json.load(open(info_path)) if os.path.exists(info_path) else {}
+    return {"workload": "1,048,576-particle dam break, 8 network steps at 4 dt + 1 plain step per cycle",
+            "cycles": cycles, "warm_steps": warm_steps, "dt": dt, "sim_time_s": net["sim_time"],
+            "plain_s": net["plain_s"], "surrogate_s": net["surrogate_s"], "speedup": net["speedup"],
+            "rel_pos_err": net["rel_err"], "meets_1pct": bool(net["rel_err"] is not None and net["rel_err"] <= 0.01),
+            "held_accel_rel_pos_err": held["rel_err"], "plain_dt_half_rel_diff": floor,
+            "judged_every_4": {"speedup": judged["speedup"], "rel_pos_err": judged["rel_err"],
+                               "fallbacks": judged["stats"]["fallbacks"], "verified": judged["stats"]["verified"],
+                               "cycle_err_mean": judged["stats"]["err_mean"], "cycle_err_max": judged["stats"]["err_max"]},
+            "training": {k: train_info.get(k) for k in ("rollouts", "particles", "rows", "steps", "batch",
+                                                        "loss_first", "loss_last", "wall_s")}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +192,10 @@ def main():
                     help="steps of the 256-scene batched sweep (0 skips it)")
     ap.add_argument("--sweep-only", action="store_true")
     ap.add_argument("--autotune-steps", type=int, default=3)
+    ap.add_argument("--surrogate-cycles", type=int, default=8, help="0 skips the surrogate leg")
+    ap.add_argument("--surrogate-warm", type=int, default=660)
+    ap.add_argument("--surrogate-model", default=os.path.join(ROOT, "ultraballpit_amd", "models",
+                                                              "dam_break_surrogate.npz"))
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -221,6 +275,9 @@ def main():
     if args.sweep_steps > 0:
         log(f"[bench] batched sweep, {args.sweep_steps} steps")
         out["sweep"] = sweep_bench(args.sweep_steps, min(args.warmup, 10), args.autotune_steps)
+    if args.surrogate_cycles > 0:
+        log(f"[bench] surrogate leg, {args.surrogate_cycles} cycles")
+        out["surrogate"] = surrogate_bench(args.surrogate_cycles, args.surrogate_warm, args.surrogate_model)
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle) on the host cores")
         out["cpu_baseline"] = cpu_baseline(sc, args.cpu_budget)

@@ -267,6 +267,56 @@ int ubp_mlp_forward(const ubp_mlp_model *m, const float *x, size_t rows, float *
  * reduction).  grad uses the parameter fields of ubp_mlp_model.  Synchronous. */
 int ubp_mlp_loss_grad(const ubp_mlp_model *m, const float *x, const float *t, size_t rows,
                       ubp_mlp_model *grad, double *loss);
+/* Surrogate rollout on an engine (one-scene handles).  Cycle: mark the
+ * current plain state (a_last, delta_last), K network steps of factor * dt
+ * (the network predicts the effective acceleration over its step), one plain
+ * step (the correction).  Every cycle records an acceleration-based error
+ * estimate est = 0.5 rms|a_net - a_plain| T_cycle^2 / rms|x - x_start|.
+ * Every `verify_every`-th cycle (0: never) is the judge: the cycle is rerun
+ * with plain WCSPH from its start, the measured position error
+ *   err = mean|x_net - x_plain| / mean|x_plain - x_start|
+ * is recorded, the plain state is kept, and err > threshold falls back to
+ * plain cycles until the next verification.  dt > 0 is a fixed plain dt,
+ * dt <= 0 the device CFL dt.  Synchronous. */
+typedef struct {
+  uint32_t cycles, plain_steps, surrogate_steps, fallbacks, verified;
+  double sim_time;                     /* simulated seconds advanced */
+  double err_last, err_max, err_mean;  /* measured error of verified cycles */
+  double est_mean, held_est_mean;      /* estimate: network, and a = a_last (no network) */
+} ubp_surrogate_stats;
+int ubp_surrogate_set_model(ubp_engine *e, const ubp_mlp_model *m);
+int ubp_surrogate_run(ubp_engine *e, int ncycles, int K, float dt, float factor, float threshold,
+                      int verify_every, ubp_surrogate_stats *stats);
+/* Building blocks: start a cycle at the current (stepped) state; one network
+ * step of size dt at cycle phase tau; the features of the current state in
+ * original order (out: n x 32; needs a step since the first mark). */
+int ubp_surrogate_mark(ubp_engine *e);
+int ubp_surrogate_step(ubp_engine *e, float dt, float tau);
+int ubp_surrogate_features(ubp_engine *e, float tau, float *out);
+
+/* Trainer: a device data set of (features, target) rows collected from
+ * engines, and Adam on the training kernels.  The target of a row is
+ * (a_eff - a_last)/|g|, a_eff = 2 (x(t + w) - x(t) - w v(t)) / w^2 being the
+ * effective acceleration over a window w of the plain trajectory. */
+typedef struct ubp_trainer ubp_trainer;
+ubp_trainer *ubp_trainer_create(size_t capacity_rows, uint64_t seed);
+void ubp_trainer_destroy(ubp_trainer *t);
+/* Rows for the particles ids[count] (original indices) of a marked engine's
+ * current state, at cycle phase tau: features now, target pending;
+ * *first_row receives the index of the first new row. */
+int ubp_trainer_collect(ubp_trainer *t, ubp_engine *e, float tau, const uint32_t *ids, size_t count,
+                        size_t *first_row);
+/* Completes pending rows [first_row, first_row + count) (same ids) from the
+ * engine's state `window` seconds after their collection. */
+int ubp_trainer_finish(ubp_trainer *t, ubp_engine *e, size_t first_row, const uint32_t *ids, size_t count,
+                       float window);
+size_t ubp_trainer_rows(const ubp_trainer *t);
+/* Start from `init` (NULL: He initialisation from the seed, normalisation
+ * from the collected rows) and reset the optimiser. */
+int ubp_trainer_init_model(ubp_trainer *t, const ubp_mlp_model *init);
+int ubp_trainer_fit(ubp_trainer *t, int steps, uint32_t batch, float lr, float lr_end, float *losses);
+int ubp_trainer_get_model(ubp_trainer *t, ubp_mlp_model *out);
+
 /* One 16x16x32 BF16 MFMA on fixed inputs against a host product: writes the
  * largest absolute difference (0 when the lane layout the kernels assume
  * holds).  Diagnostic, synchronous. */

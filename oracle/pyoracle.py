@@ -80,6 +80,7 @@ def lib():
     L.orc_timestep.argtypes = [C.c_void_p]
     L.orc_timestep.restype = C.c_double
     L.orc_step_auto.argtypes = [C.c_void_p, C.c_int, dp]
+    L.orc_step_features.argtypes = [C.c_void_p, C.c_double, fp, C.c_float, fp]
     L.orc_get.argtypes = [C.c_void_p, C.c_int, dp]
     L.orc_get_boundary_pressure.argtypes = [C.c_void_p, dp]
     L.orc_error_metrics.argtypes = [C.c_void_p, dp]
@@ -210,6 +211,15 @@ class Oracle:
         dts = np.zeros(nsteps, np.float64)
         lib().orc_step_auto(self.h, int(nsteps), _dp(dts))
         return dts
+
+    def step_features(self, aux, tau: float, dt: float = 0.0) -> np.ndarray:
+        """One step (dt <= 0: CFL dt) returning the surrogate features of the
+        state after its force pass (n x 32); aux: n x 4 (a_last, delta_last)."""
+        a = np.ascontiguousarray(aux, np.float32).reshape(self.n, 4)
+        out = np.zeros((self.n, 32), np.float32)
+        lib().orc_step_features(self.h, float(dt), a.ctypes.data_as(C.POINTER(C.c_float)), float(tau),
+                                out.ctypes.data_as(C.POINTER(C.c_float)))
+        return out
 
     def get(self, field: str) -> np.ndarray:
         out = np.empty(self.n, np.float64)

@@ -384,6 +384,10 @@ template <class R> struct Sim {
   State<R> ckpt;
   Grid<R> ckpt_grid;
   std::vector<uint32_t> last_nbr_count;
+  // surrogate feature hook: computed inside step() after the forces
+  const float* feat_aux = nullptr;
+  float feat_tau = 0.0f;
+  float* feat_out = nullptr;
 
   static R rest_density(uint8_t ft) { return ft == 0 ? R(1000.0) : R(1.204); }  // eos.rs:7-10
   static R gamma_of(uint8_t ft) { return ft == 0 ? R(7.0) : R(1.4); }            // eos.rs:13-19
@@ -670,10 +674,82 @@ template <class R> struct Sim {
       if (p.z[i] > dmax[2]) { p.z[i] = dmax[2]; if (p.vz[i] > R(0)) p.vz[i] = -rest * p.vz[i]; }
     }
     compute_forces(dt);
+    if (feat_out) features(feat_aux, R(feat_tau), feat_out);
     for (size_t i = 0; i < n; ++i) {
       p.vx[i] += p.ax[i] * half_dt;
       p.vy[i] += p.ay[i] * half_dt;
       p.vz[i] += p.az[i] * half_dt;
+    }
+  }
+
+  // Surrogate input features of the engine's feature pass
+  // (ultraballpit_amd/csrc/ubp_kernels.hip k_feat), same neighbour order and
+  // FP32 operation order: kernel-weighted sums over the radius-2h neighbours
+  // of volumes, gradients, relative velocities, last-correction density
+  // deviations and accelerations (aux = a_last xyz, delta_last per particle).
+  void features(const float* aux, R tau, float* out) const {
+    const size_t n = p.size();
+    const R sr = R(2) * h;
+    const R inv_u = cs > R(0) ? R(10) / cs : R(1);
+    const R g2 = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+    const R inv_g = R(1) / (g2 > R(1e-12) ? std::sqrt(g2) : R(1));
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long li = 0; li < long(n); ++li) {
+      const size_t i = size_t(li);
+      const R ir0 = R(1) / rest_density(p.ft[i]);
+      const R avx = aux[4 * i], avy = aux[4 * i + 1], avz = aux[4 * i + 2], dli = aux[4 * i + 3];
+      R s0 = p.m[i] * ir0 * wendland<R>(R(0), h);
+      R s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0, s6 = 0, s7 = 0, s8 = 0, s9 = 0, s10 = 0, s11 = 0, s12 = 0, s13 = 0;
+      grid.for_each(i, p.x, p.y, p.z, sr, [&](size_t j) {
+        const R dx = p.x[i] - p.x[j];
+        const R dy = p.y[i] - p.y[j];
+        const R dz = p.z[i] - p.z[j];
+        const R dist_sq = dx * dx + dy * dy + dz * dz;
+        const R r = std::sqrt(dist_sq);
+        const R w = wendland<R>(r, h);
+        R gx, gy, gz;
+        wendland_grad(dx, dy, dz, r, h, gx, gy, gz);
+        const R vol = p.m[j] * ir0;
+        const R vw = vol * w;
+        s0 += vw;
+        s1 += vol * gx;
+        s2 += vol * gy;
+        s3 += vol * gz;
+        const R dvx = p.vx[j] - p.vx[i], dvy = p.vy[j] - p.vy[i], dvz = p.vz[j] - p.vz[i];
+        s4 += vw * dvx;
+        s5 += vw * dvy;
+        s6 += vw * dvz;
+        s7 += vol * (dvx * gx + dvy * gy + dvz * gz);
+        const R dd = R(aux[4 * j + 3]) - dli;
+        s8 += vol * dd * gx;
+        s9 += vol * dd * gy;
+        s10 += vol * dd * gz;
+        s11 += vw * (R(aux[4 * j]) - avx);
+        s12 += vw * (R(aux[4 * j + 1]) - avy);
+        s13 += vw * (R(aux[4 * j + 2]) - avz);
+      });
+      R sw = R(2), wx = R(0), wy = R(0), wz = R(0);
+      R d, wnx, wny, wnz;
+      if (sdf.wall(p.x[i], p.y[i], p.z[i], d, wnx, wny, wnz)) {
+        const R s = d / h;
+        if (s < R(2)) {
+          s0 += wall_omega<R>(s);
+          sw = s < R(0) ? R(0) : s;
+          const R sc = R(1) - R(0.5) * sw;
+          wx = wnx * sc;
+          wy = wny * sc;
+          wz = wnz * sc;
+        }
+      }
+      const R d0 = s0 - R(1);
+      const R o[32] = {d0, s1 * h, s2 * h, s3 * h,
+                       s4 * inv_u, s5 * inv_u, s6 * inv_u, s7 * h * inv_u,
+                       s8 * h * R(100), s9 * h * R(100), s10 * h * R(100), s11 * inv_g,
+                       s12 * inv_g, s13 * inv_g, p.vx[i] * inv_u, p.vy[i] * inv_u,
+                       p.vz[i] * inv_u, avx * inv_g, avy * inv_g, avz * inv_g,
+                       dli * R(100), (d0 - dli) * R(100), sw, wx,
+                       wy, wz, tau, R(0), R(0), R(0), R(0), R(0)};
+      for (int k = 0; k < 32; ++k) out[32 * i + k] = float(o[k]);
     }
   }
 
@@ -856,6 +932,22 @@ void orc_step_auto(void* h, int nsteps, double* dts) {
     if (dts) dts[k] = dt;
     orc_step(h, dt);
   }
+}
+
+// One step (dt <= 0: dt = compute_timestep) that also writes the surrogate
+// features of the state after its force pass (out: n x 32; aux: n x 4).
+void orc_step_features(void* h, double dt, const float* aux, float tau, float* out) {
+  Handle* hd = static_cast<Handle*>(h);
+  if (dt <= 0.0) dt = orc_timestep(h);
+  auto run = [&](auto* s, auto d) {
+    s->feat_aux = aux;
+    s->feat_tau = tau;
+    s->feat_out = out;
+    s->step(d);
+    s->feat_out = nullptr;
+    s->feat_aux = nullptr;
+  };
+  if (hd->d) run(hd->d, dt); else run(hd->f, float(dt));
 }
 
 // field ids: 0 x 1 y 2 z 3 vx 4 vy 5 vz 6 ax 7 ay 8 az 9 rho 10 p 11 m 12 T

@@ -1,0 +1,150 @@
+"""Learned surrogate on the engine: features bit-exact against the oracle's
+restatement, data collection, GPU training, the fallback controller, and the
+gain of the trained network over the same cycle without it (held
+acceleration a = a_last)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import ultraballpit_amd as ubp
+from ultraballpit_amd import scenes, surrogate
+from parity_util import first_mismatch, oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not ubp.gpu_available():
+        pytest.fail("no gfx950 device / engine library not loadable")
+
+
+def test_features_bitwise_vs_oracle():
+    sc = scenes.dam_break_small(lat=(12, 12, 24))
+    say("case: surrogate features, dam_break_small 3456, mark after 5 steps, 3 more steps")
+    e = ubp.HipKernel.from_scene(sc)
+    o = oracle(sc)
+    e.step_n(5, use_graph=False)
+    o.step_auto(5)
+    e.surrogate_mark()
+    f32 = np.float32
+    rho = o.get("density").astype(f32)
+    aux = np.stack([o.get("ax").astype(f32), o.get("ay").astype(f32), o.get("az").astype(f32),
+                    rho / f32(1000.0) - f32(1.0)], axis=1)
+    e.step_n(3, use_graph=False)
+    o.step_auto(2)
+    fo = o.step_features(aux, 0.375)
+    fe = e.surrogate_features(0.375)
+    bad = np.nonzero(fe.view(np.uint32) != fo.view(np.uint32))
+    assert bad[0].size == 0, f"{bad[0].size} feature values differ, first (particle, feature) = " \
+                             f"({bad[0][0]}, {bad[1][0]}): {fe[bad][0]} vs {fo[bad][0]}"
+    assert np.isfinite(fe).all() and np.abs(fe[:, :27]).max() > 0
+
+
+def test_mark_does_not_change_plain_steps():
+    sc = scenes.dam_break_small(lat=(10, 10, 20))
+    say("case: plain steps with surrogate state carried == plain steps")
+    a = ubp.HipKernel.from_scene(sc)
+    b = ubp.HipKernel.from_scene(sc)
+    a.step_n(4, use_graph=False)
+    b.step_n(4, use_graph=False)
+    a.surrogate_mark()
+    a.step_n(12, use_graph=True)
+    b.step_n(12, use_graph=True)
+    assert first_mismatch(a.particles(), b.particles()) is None
+
+
+def test_collect_rows_are_features_and_targets():
+    sc = scenes.dam_break_small(lat=(10, 10, 20))
+    say("case: trainer rows == engine features / effective-acceleration targets")
+    e = ubp.HipKernel.from_scene(sc)
+    dt = e.dt()
+    e.step(dt)
+    e.surrogate_mark()
+    m0 = e.particles()
+    for _ in range(4):
+        e.step(dt)
+    tr = ubp.Trainer(1000, seed=3)
+    ids = np.array([0, 5, 17, 999, 1500, 1999], np.uint32)
+    first = tr.collect(e, ids, 0.5)
+    assert first == 0 and tr.rows() == len(ids)
+    feats = e.surrogate_features(0.5)
+    p0 = e.particles()
+    with pytest.raises(ubp.EngineError):
+        tr.init_model(None)  # targets pending
+    for _ in range(4):
+        e.step(dt)
+    p1 = e.particles()
+    w = np.float32(4) * np.float32(dt)
+    tr.finish(e, first, ids, float(w))
+    tr.init_model(None)
+    m = tr.model()
+    # normalisation stats are the column means of the collected rows
+    assert np.allclose(m["in_mean"], feats[ids].mean(axis=0), rtol=1e-5, atol=1e-6)
+    f64 = np.float64
+    aeff = np.stack([2.0 * ((p1[x][ids].astype(f64) - p0[x][ids]) - f64(w) * p0[v][ids]) / f64(w) ** 2
+                     for x, v in (("x", "vx"), ("y", "vy"), ("z", "vz"))], axis=1)
+    want_t = (aeff - np.stack([m0[k][ids] for k in ("ax", "ay", "az")], axis=1)) / 9.81
+    assert np.allclose(m["out_mean"], want_t.mean(axis=0), rtol=1e-3, atol=1e-4)
+    tr.close()
+
+
+def test_training_reduces_loss():
+    sc = scenes.dam_break_small(lat=(16, 16, 32))
+    say("case: train 400 steps on 4 cycles of a 8192-particle dam break")
+    model, losses, info = surrogate.train([sc], cycles=4, steps=400, batch=8192, lr=2e-3)
+    say("  ", info)
+    assert np.isfinite(losses).all()
+    assert losses[-20:].mean() < 0.5 * losses[:20].mean()
+
+
+def test_rejecting_every_cycle_reproduces_the_plain_run():
+    sc = scenes.dam_break_small(lat=(10, 10, 20))
+    say("case: verify every cycle, threshold -1: every cycle falls back; state == plain run")
+    model = surrogate.zero_model(ubp.load_model(MODEL_PATH) if _has_model() else _tiny_model())
+    dt = surrogate.fixed_dt(sc)
+    K, factor, cycles = 4, 2, 3
+    s = ubp.HipKernel.from_scene(sc)
+    s.surrogate_set_model(model)
+    st = s.surrogate_run(cycles, K, dt, factor, threshold=-1.0, verify_every=1)
+    assert st["fallbacks"] == cycles and st["cycles"] == cycles and st["verified"] == cycles
+    p = ubp.HipKernel.from_scene(sc)
+    for _ in range(1 + cycles * (K * factor + 1)):
+        p.step(dt)
+    # the engine accumulates the FP32 dt it steps with
+    assert st["sim_time"] == pytest.approx((1 + cycles * (K * factor + 1)) * float(np.float32(dt)), rel=1e-9)
+    assert first_mismatch(s.particles(), p.particles()) is None
+
+
+def test_trained_network_gains_over_held_acceleration():
+    train_sc = [scenes.dam_break_small(lat=(20, 20, 40), seed=s) for s in (11, 12)]
+    test_sc = scenes.dam_break_small(lat=(20, 20, 40), seed=99)
+    say("case: gain test: train on 2 seeds, evaluate a held-out seed, 6 cycles K=8 at 4 dt")
+    model, losses, info = surrogate.train(train_sc, cycles=6, steps=1500, batch=16384, lr=2e-3)
+    say("  train", info)
+    net = surrogate.evaluate(test_sc, model, cycles=6)
+    held = surrogate.evaluate(test_sc, surrogate.zero_model(model), cycles=6)
+    say(f"  network: rel err {net['rel_err']:.4f} est {net['stats']['est_mean']:.4f}; "
+        f"held: rel err {held['rel_err']:.4f} est {held['stats']['est_mean']:.4f}")
+    assert net["rel_err"] < held["rel_err"]
+
+
+MODEL_PATH = os.path.join(ubp.HERE, "models", "dam_break_surrogate.npz")
+
+
+def _has_model():
+    return os.path.exists(MODEL_PATH)
+
+
+def _tiny_model():
+    m = {k: np.zeros(s, np.float32) for k, s in ubp.MLP_FIELDS}
+    m["in_inv_std"][:] = 1.0
+    m["out_std"][:] = 1.0
+    return m

@@ -112,6 +112,13 @@ def _model_from_c(c: _MlpModel) -> dict:
     return {k: np.ctypeslib.as_array(getattr(c, k)).astype(np.float32).reshape(s).copy() for k, s in MLP_FIELDS}
 
 
+class _SurStats(C.Structure):
+    _fields_ = [("cycles", C.c_uint32), ("plain_steps", C.c_uint32), ("surrogate_steps", C.c_uint32),
+                ("fallbacks", C.c_uint32), ("verified", C.c_uint32), ("sim_time", C.c_double),
+                ("err_last", C.c_double), ("err_max", C.c_double), ("err_mean", C.c_double),
+                ("est_mean", C.c_double), ("held_est_mean", C.c_double)]
+
+
 class _Scene(C.Structure):
     _fields_ = [("particles", C.POINTER(_ParticlesIn)), ("h", C.c_float), ("gravity", C.c_float * 3),
                 ("speed_of_sound", C.c_float), ("cfl_number", C.c_float),
@@ -131,7 +138,10 @@ EXPORTS = ("ubp_create", "ubp_destroy", "ubp_last_error", "ubp_gpu_available",
            "ubp_step_n_profiled", "ubp_calibration_copy", "ubp_create_batch", "ubp_scene_count",
            "ubp_scene_range", "ubp_get_scene_dts", "ubp_scene_error_metrics", "ubp_autotune",
            "ubp_get_tuning", "ubp_set_tuning_ex", "ubp_mlp_forward", "ubp_mlp_loss_grad",
-           "ubp_mfma_selftest")
+           "ubp_mfma_selftest", "ubp_surrogate_set_model", "ubp_surrogate_run", "ubp_surrogate_mark",
+           "ubp_surrogate_step", "ubp_surrogate_features", "ubp_trainer_create", "ubp_trainer_destroy",
+           "ubp_trainer_collect", "ubp_trainer_finish", "ubp_trainer_rows", "ubp_trainer_init_model", "ubp_trainer_fit",
+           "ubp_trainer_get_model")
 
 #: kernel order of ubp_step_n_profiled
 KERNELS = ("k_kick", "k_xsph", "k_drift", "k_scan", "k_scatter", "k_rank_gather", "k_neigh", "k_expand",
@@ -194,6 +204,22 @@ def lib():
     L.ubp_mlp_loss_grad.argtypes = [C.POINTER(_MlpModel), fp, fp, C.c_size_t, C.POINTER(_MlpModel),
                                     C.POINTER(C.c_double)]
     L.ubp_mfma_selftest.argtypes = [C.POINTER(C.c_double)]
+    L.ubp_surrogate_set_model.argtypes = [C.c_void_p, C.POINTER(_MlpModel)]
+    L.ubp_surrogate_run.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_int,
+                                    C.POINTER(_SurStats)]
+    L.ubp_surrogate_mark.argtypes = [C.c_void_p]
+    L.ubp_surrogate_step.argtypes = [C.c_void_p, C.c_float, C.c_float]
+    L.ubp_surrogate_features.argtypes = [C.c_void_p, C.c_float, fp]
+    L.ubp_trainer_create.restype = C.c_void_p
+    L.ubp_trainer_create.argtypes = [C.c_size_t, C.c_uint64]
+    L.ubp_trainer_destroy.argtypes = [C.c_void_p]
+    L.ubp_trainer_collect.argtypes = [C.c_void_p, C.c_void_p, C.c_float, u32p, C.c_size_t, C.POINTER(C.c_size_t)]
+    L.ubp_trainer_finish.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, u32p, C.c_size_t, C.c_float]
+    L.ubp_trainer_rows.restype = C.c_size_t
+    L.ubp_trainer_rows.argtypes = [C.c_void_p]
+    L.ubp_trainer_init_model.argtypes = [C.c_void_p, C.POINTER(_MlpModel)]
+    L.ubp_trainer_fit.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_float, C.c_float, fp]
+    L.ubp_trainer_get_model.argtypes = [C.c_void_p, C.POINTER(_MlpModel)]
     for name in EXPORTS:
         getattr(L, name)
     _lib = L
@@ -371,6 +397,32 @@ class HipKernel:
     def scene_count(self) -> int:
         return int(lib().ubp_scene_count(self._h))
 
+    # -- learned surrogate ----------------------------------------------------
+    def surrogate_set_model(self, model: dict):
+        mc = _model_to_c(model)
+        _check(lib().ubp_surrogate_set_model(self._h, C.byref(mc)), "surrogate_set_model")
+
+    def surrogate_run(self, ncycles: int, K: int = 8, dt: float = 0.0, factor: float = 4.0,
+                      threshold: float = 0.01, verify_every: int = 0) -> dict:
+        """Correction cycles of K network steps at factor x dt plus one plain
+        step each; every verify_every-th cycle is rerun with plain WCSPH as
+        the judge (see ubp.h)."""
+        s = _SurStats()
+        _check(lib().ubp_surrogate_run(self._h, int(ncycles), int(K), float(dt), float(factor),
+                                       float(threshold), int(verify_every), C.byref(s)), "surrogate_run")
+        return {k: getattr(s, k) for k, _ in _SurStats._fields_}
+
+    def surrogate_mark(self):
+        _check(lib().ubp_surrogate_mark(self._h), "surrogate_mark")
+
+    def surrogate_step(self, dt: float, tau: float):
+        _check(lib().ubp_surrogate_step(self._h, float(dt), float(tau)), "surrogate_step")
+
+    def surrogate_features(self, tau: float) -> np.ndarray:
+        out = np.zeros((self._n, MLP_IN), np.float32)
+        _check(lib().ubp_surrogate_features(self._h, float(tau), _fp(out)), "surrogate_features")
+        return out
+
     # -- SimulationKernel ----------------------------------------------------
     def step(self, dt: float):
         _check(lib().ubp_step(self._h, float(dt)), "step")
@@ -521,6 +573,68 @@ class HipBatch(HipKernel):
         m = _Metrics()
         _check(lib().ubp_scene_error_metrics(self._h, int(s), C.byref(m)), "scene_error_metrics")
         return ErrorMetrics(m.max_density_variation, m.energy_conservation, m.mass_conservation)
+
+
+class Trainer:
+    """Device data set of (features, (a - a_last)/|g|) rows and the Adam
+    training loop on the BF16-MFMA kernels."""
+
+    def __init__(self, capacity_rows: int, seed: int = 1):
+        self._h = lib().ubp_trainer_create(int(capacity_rows), int(seed))
+        if not self._h:
+            raise GpuInitError(_err())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ubp_trainer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def collect(self, engine: HipKernel, ids, tau: float) -> int:
+        """Features of ids now (target pending); returns the first row."""
+        ids = np.ascontiguousarray(ids, np.uint32)
+        first = C.c_size_t(0)
+        _check(lib().ubp_trainer_collect(self._h, engine._h, float(tau), _u32p(ids), ids.size, C.byref(first)),
+               "trainer_collect")
+        return int(first.value)
+
+    def finish(self, engine: HipKernel, first: int, ids, window: float):
+        """Targets of pending rows from the engine state `window` seconds later."""
+        ids = np.ascontiguousarray(ids, np.uint32)
+        _check(lib().ubp_trainer_finish(self._h, engine._h, int(first), _u32p(ids), ids.size, float(window)),
+               "trainer_finish")
+
+    def rows(self) -> int:
+        return int(lib().ubp_trainer_rows(self._h))
+
+    def init_model(self, model: dict | None = None):
+        mc = _model_to_c(model) if model is not None else None
+        _check(lib().ubp_trainer_init_model(self._h, C.byref(mc) if mc is not None else None), "trainer_init_model")
+
+    def fit(self, steps: int, batch: int = 65536, lr: float = 1e-3, lr_end: float = 1e-4) -> np.ndarray:
+        losses = np.zeros(max(1, steps), np.float32)
+        _check(lib().ubp_trainer_fit(self._h, int(steps), int(batch), float(lr), float(lr_end), _fp(losses)),
+               "trainer_fit")
+        return losses[:steps]
+
+    def model(self) -> dict:
+        mc = _MlpModel()
+        _check(lib().ubp_trainer_get_model(self._h, C.byref(mc)), "trainer_get_model")
+        return _model_from_c(mc)
+
+
+def save_model(path: str, model: dict):
+    np.savez(path, **{k: np.asarray(v, np.float32) for k, v in model.items()})
+
+
+def load_model(path: str) -> dict:
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k].astype(np.float32) for k, _ in MLP_FIELDS}
 
 
 def mlp_forward(model: dict, x) -> np.ndarray:

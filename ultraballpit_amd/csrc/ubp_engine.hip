@@ -136,9 +136,10 @@ struct Snapshot {
   uint8_t* ft = nullptr;
   uint32_t *orig = nullptr, *start = nullptr;
   uint16_t* sid = nullptr;
+  float4* aux = nullptr;  // surrogate last-correction state
   void release() {
     dfree(pos); dfree(vel); dfree(acc); dfree(wall); dfree(rho); dfree(prs);
-    dfree(ft); dfree(orig); dfree(start); dfree(sid);
+    dfree(ft); dfree(orig); dfree(start); dfree(sid); dfree(aux);
     valid = false;
   }
 };
@@ -201,6 +202,22 @@ struct Engine {
   // checkpoint (reference save/restore_checkpoint) and autotuner snapshot
   Snapshot ck, tune_ck;
 
+  // learned surrogate: last-correction state (gathered with the particles),
+  // feature record and features, judge buffers, the network
+  bool sur_on = false;
+  float4* aux[2] = {nullptr, nullptr};
+  float4* sr = nullptr;
+  float* feat = nullptr;
+  float4* x0 = nullptr;
+  double* jpart = nullptr;
+  float* dts = nullptr;
+  uint32_t* inv = nullptr;
+  float4* dv0 = nullptr;   // zero drift correction (network steps have no XSPH pass)
+  float4* xref = nullptr;  // positions by original id (verification)
+  MlpBuf model;
+  bool has_model = false;
+  Snapshot sur_ck;
+
   // host mirror (original order)
   std::vector<float> h_temp;
   std::vector<uint8_t> h_ft;
@@ -234,6 +251,10 @@ struct Engine {
     dfree(d_sp); dfree(sid[0]); dfree(sid[1]); dfree(d_soff); dfree(sacc);
     ck.release();
     tune_ck.release();
+    sur_ck.release();
+    dfree(aux[0]); dfree(aux[1]); dfree(sr); dfree(feat); dfree(x0); dfree(jpart); dfree(dts); dfree(inv);
+    dfree(dv0); dfree(xref);
+    model.release();
     if (h_stage) { (void)hipHostFree(h_stage); h_stage = nullptr; }
     if (st) { (void)hipStreamDestroy(st); st = nullptr; }
   }
@@ -655,7 +676,8 @@ struct Engine {
     launch(PASS_ELEM, k_scatter, n, P, key, arr, start, orig[pp], stage_src, stage_orig);
     rec(E, 5);
     UBP_LAUNCH_M(PASS_ELEM, k_rank_gather, n, stage_src, stage_orig, key, start, pos_t, vel_t, rho_n, wall_t,
-                 ft[pp], pos, vel, dr, wall, ft[pp ^ 1], orig[pp ^ 1], sid[pp], sid[pp ^ 1]);
+                 ft[pp], pos, vel, dr, wall, ft[pp ^ 1], orig[pp ^ 1], sid[pp], sid[pp ^ 1],
+                 (const float4*)(sur_on ? aux[pp] : nullptr), aux[pp ^ 1], sr);
     pp ^= 1;
   }
 
@@ -924,6 +946,7 @@ struct Engine {
       dalloc(&S.ft, n); dalloc(&S.orig, n); dalloc(&S.start, size_t(P.ncodes) + 1);
       if (multi) dalloc(&S.sid, n);
     }
+    if (sur_on && !S.aux) dalloc(&S.aux, n);
     auto cp = [&](void* d, const void* s, size_t b) { if (b) HIPCHK(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, st)); };
     const bool pre = needs_init;
     cp(S.pos, pre ? pos_t : pos, n * 16);
@@ -936,6 +959,7 @@ struct Engine {
     cp(S.orig, orig[pp], n * 4);
     cp(S.start, start, (size_t(P.ncodes) + 1) * 4);
     if (multi) cp(S.sid, sid[pp], n * 2);
+    if (sur_on) cp(S.aux, aux[pp], n * 16);
     S.needs_init = needs_init;
     S.pp = pp;
     S.valid = true;
@@ -956,6 +980,7 @@ struct Engine {
     cp(orig[pp], S.orig, n * 4);
     cp(start, S.start, (size_t(P.ncodes) + 1) * 4);
     if (multi) cp(sid[pp], S.sid, n * 2);
+    if (sur_on && S.aux) cp(aux[pp], S.aux, n * 16);
     if (!needs_init) build_neighbors();  // neighbour lists of the restored state
     dt_valid = false;
     invalidate_graph();
@@ -1024,6 +1049,259 @@ struct Engine {
     stepped = was_stepped;
   }
 
+  // ---- learned surrogate ----
+  float gmag() const {
+    const float g2 = P.g[0] * P.g[0] + P.g[1] * P.g[1] + P.g[2] * P.g[2];
+    return g2 > 1e-12f ? std::sqrt(g2) : 1.0f;
+  }
+  float inv_u() const { return P.cs > 0.0f ? 10.0f / P.cs : 1.0f; }
+
+  void sur_enable() {
+    if (multi) throw Error(UBP_EINVAL, "the surrogate runs on one-scene engines");
+    if (sur_on) return;
+    HIPCHK(hipStreamSynchronize(st));
+    for (int k = 0; k < 2; ++k) { dalloc(&aux[k], n); HIPCHK(hipMemset(aux[k], 0, n * 16)); }
+    dalloc(&sr, 3 * n);
+    HIPCHK(hipMemset(sr, 0, 3 * n * 16));
+    dalloc(&feat, 32 * n);
+    dalloc(&x0, n);
+    dalloc(&jpart, 3 * ((n + 255) / 256 + 1));
+    dalloc(&dts, 4);
+    dalloc(&inv, n);
+    dalloc(&dv0, n);
+    HIPCHK(hipMemset(dv0, 0, n * 16));
+    dalloc(&xref, n);
+    sur_on = true;
+    invalidate_graph();  // the gather now carries the surrogate state
+  }
+
+  void sur_set_model(const ubp_mlp_model* m) {
+    model.upload(m, st);
+    has_model = true;
+  }
+
+  // cycle start: a_last := a, delta_last := rho / rho0 - 1, x0 := x
+  void sur_mark() {
+    sur_enable();
+    if (needs_init) throw Error(UBP_EINVAL, "the surrogate needs a stepped state (no step yet)");
+    launch(PASS_ELEM, k_aux_set, n, P, (const float4*)acc, (const float*)rho_n, (const float4*)pos,
+           (const uint8_t*)ft[pp], (const uint32_t*)orig[pp], aux[pp], sr, x0);
+  }
+
+  void sur_features(float tau, float* rho_out) {
+    launch(PASS_DENS, k_feat, n, P, (const float4*)sr, (const float4*)wall, (const uint8_t*)ft[pp],
+           (const uint32_t*)start, (const uint32_t*)nbr, (const uint32_t*)nn, tau, inv_u(), 1.0f / gmag(), feat,
+           rho_out);
+  }
+
+  void sur_predict(float4* out) {
+    if (!has_model) throw Error(UBP_EINVAL, "no surrogate model set");
+    hipLaunchKernelGGL(k_mlp_fwd<1>, dim3(mlp_fwd_grid(n)), dim3(256), 0, st, model.view(), (const float*)feat,
+                       uint32_t(n), (float*)nullptr, (const float4*)aux[pp], out, gmag());
+    LAUNCHCHK("k_mlp_fwd");
+  }
+
+  // One KDK step with the network's acceleration; dt in dts.  The network
+  // predicts the effective acceleration over the step (it absorbs the XSPH
+  // drift correction, so there is no XSPH pass); the same binning,
+  // neighbour lists and walls as the plain step, with the density and force
+  // passes replaced by the feature pass and the network.
+  void sur_step(float tau, hipEvent_t* E = nullptr) {
+    rec(E, 0);
+    UBP_LAUNCH_M(PASS_ELEM, k_kick, n, vel, acc, pos, rho_n, dts, xr);
+    rec(E, 1);
+    rec(E, 2);
+    UBP_LAUNCH_M(PASS_ELEM, k_drift, n, pos, vel, dv0, dts, pos_t, vel_t, wall_t, key, arr, cnt);
+    rec(E, 3);
+    bin(E);
+    rec(E, 6);
+    build_neighbors(E);
+    rec(E, 8);
+    sur_features(tau, rho_n);
+    rec(E, 9);
+    sur_predict(acc);
+    rec(E, 10);
+    UBP_LAUNCH_M(PASS_ELEM, k_kick2_cfl, n, vel, acc, dts, red, sacc, 1, 0);
+    rec(E, 11);
+  }
+
+  struct Judge { double err2, held2, disp2; };
+  // network prediction at the current (plain) state vs the plain acceleration
+  Judge sur_judge(float tau) {
+    sur_features(tau, nullptr);
+    sur_predict(dv);
+    const uint32_t g = uint32_t((n + 255) / 256);
+    hipLaunchKernelGGL(k_judge, dim3(g), dim3(256), 0, st, P, (const float4*)acc, (const float4*)dv,
+                       (const float4*)aux[pp], (const float4*)pos, (const uint32_t*)orig[pp], (const float4*)x0,
+                       jpart);
+    LAUNCHCHK("k_judge");
+    std::vector<double> h(3 * size_t(g));
+    HIPCHK(hipMemcpyAsync(h.data(), jpart, h.size() * 8, hipMemcpyDeviceToHost, st));
+    sync();
+    Judge J{0.0, 0.0, 0.0};
+    for (uint32_t b = 0; b < g; ++b) { J.err2 += h[3 * b]; J.held2 += h[3 * b + 1]; J.disp2 += h[3 * b + 2]; }
+    return J;
+  }
+
+  float read_dt() {
+    float v = 0.0f;
+    HIPCHK(hipMemcpyAsync(&v, dtp, 4, hipMemcpyDeviceToHost, st));
+    sync();
+    return v;
+  }
+
+  // Correction cycles: [mark, K network steps of factor * dt, one plain step].
+  // Each cycle records an acceleration-based error estimate
+  //   est = 0.5 rms|a_net - a_plain| T_cycle^2 / rms|x - x_cycle_start|
+  // (and the same for a = a_last).  Every `verify_every`-th cycle (0: never)
+  // is also rerun with plain steps from the cycle start: the measured error
+  //   err = mean|x_net - x_plain| / mean|x_plain - x_cycle_start|
+  // judges the network; the plain rerun's state is kept, and err > threshold
+  // switches to plain cycles until the next verification is due.
+  // dt > 0: fixed plain dt; dt <= 0: the device CFL dt.
+  void sur_run(int ncycles, int K, float dt, float factor, float thr, int verify_every, ubp_surrogate_stats* S) {
+    if (!has_model) throw Error(UBP_EINVAL, "no surrogate model set");
+    if (K < 1 || factor <= 0.0f || ncycles < 0 || verify_every < 0)
+      throw Error(UBP_EINVAL, "bad surrogate cycle parameters");
+    sur_enable();
+    std::memset(S, 0, sizeof(*S));
+    stepped = true;
+    const int nplain_cycle = int(std::lround(double(K) * factor)) + 1;
+    auto plain = [&](int k) -> double {
+      double t = 0.0;
+      for (int s = 0; s < k; ++s) {
+        if (dt > 0.0f) {
+          step_host_dt(dt);
+          t += dt;
+        } else {
+          ensure_dt();
+          t += read_dt();
+          step_device_dt(1, false);
+        }
+      }
+      S->plain_steps += uint32_t(k);
+      return t;
+    };
+    if (needs_init) S->sim_time += plain(1);
+    double est_sum = 0.0, held_sum = 0.0, err_sum = 0.0;
+    int cool = 0;
+    for (int c = 0; c < ncycles; ++c) {
+      S->cycles += 1;
+      if (cool > 0) {  // fallback: plain cycle
+        --cool;
+        S->sim_time += plain(nplain_cycle);
+        continue;
+      }
+      const bool verify = verify_every > 0 && (c % verify_every) == 0;
+      sur_mark();
+      if (verify) save_snap(sur_ck);
+      float dtc = dt;
+      if (dt > 0.0f) {
+        hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(64), 0, st, dts, factor * dt, 1u);
+      } else {
+        ensure_dt();
+        dtc = read_dt();
+        hipLaunchKernelGGL(k_scale_dt, dim3(1), dim3(64), 0, st, dts, (const float*)dtp, factor);
+      }
+      LAUNCHCHK("surrogate dt");
+      // the first network step starts from the network's acceleration at
+      // the marked state (a_last itself is instantaneous, not effective)
+      sur_features(0.0f, nullptr);
+      sur_predict(acc);
+      for (int s = 1; s <= K; ++s) sur_step(float(s) / float(K));
+      S->surrogate_steps += uint32_t(K);
+      const double tnet = double(K) * double(factor) * double(dtc);
+      const double tc = tnet + plain(1);
+      const Judge J = sur_judge(1.0f);
+      const double nd = double(n);
+      const double disp = std::sqrt(J.disp2 / nd);
+      if (disp > 0.0) {
+        est_sum += 0.5 * std::sqrt(J.err2 / nd) * tc * tc / disp;
+        held_sum += 0.5 * std::sqrt(J.held2 / nd) * tc * tc / disp;
+      }
+      if (verify) {
+        const uint32_t g = uint32_t((n + 255) / 256);
+        hipLaunchKernelGGL(k_store_pos, dim3(g), dim3(256), 0, st, uint32_t(n), (const float4*)pos,
+                           (const uint32_t*)orig[pp], xref);
+        LAUNCHCHK("k_store_pos");
+        restore_snap(sur_ck);
+        plain(nplain_cycle);
+        hipLaunchKernelGGL(k_poserr, dim3(g), dim3(256), 0, st, uint32_t(n), (const float4*)pos,
+                           (const uint32_t*)orig[pp], (const float4*)xref, (const float4*)x0, jpart);
+        LAUNCHCHK("k_poserr");
+        std::vector<double> h(2 * size_t(g));
+        HIPCHK(hipMemcpyAsync(h.data(), jpart, h.size() * 8, hipMemcpyDeviceToHost, st));
+        sync();
+        double se = 0.0, sd = 0.0;
+        for (uint32_t b = 0; b < g; ++b) { se += h[2 * b]; sd += h[2 * b + 1]; }
+        const double err = sd > 0.0 ? se / sd : 0.0;
+        S->verified += 1;
+        S->err_last = err;
+        S->err_max = std::max(S->err_max, err);
+        err_sum += err;
+        if (err > double(thr)) {
+          S->fallbacks += 1;
+          cool = verify_every > 1 ? verify_every - 1 : 0;
+        }
+      }
+      S->sim_time += tc;
+    }
+    const uint32_t judged = S->surrogate_steps / uint32_t(K);
+    if (judged) { S->est_mean = est_sum / judged; S->held_est_mean = held_sum / judged; }
+    if (S->verified) S->err_mean = err_sum / S->verified;
+    sur_ck.release();
+    sync();
+  }
+
+  // training rows, first half: features + (x, v, a_last) of original ids[]
+  size_t collect(Trainer& T, float tau, const uint32_t* ids, size_t count) {
+    if (!sur_on || needs_init) throw Error(UBP_EINVAL, "collect needs a marked, stepped engine");
+    if (T.dev != dev) throw Error(UBP_EINVAL, "trainer and engine on different devices");
+    if (T.rows + count > T.cap) throw Error(UBP_EOVERFLOW, "trainer capacity exceeded");
+    const size_t first = T.rows;
+    if (count == 0) return first;
+    for (size_t k = 0; k < count; ++k)
+      if (ids[k] >= n) throw Error(UBP_EINVAL, "particle id out of range");
+    sur_features(tau, nullptr);
+    launch(PASS_ELEM, k_inverse, n, (const uint32_t*)orig[pp], uint32_t(n), inv);
+    DevArr<uint32_t> d_ids(count);
+    HIPCHK(hipMemcpyAsync(d_ids.p, ids, count * 4, hipMemcpyHostToDevice, st));
+    launch(PASS_ELEM, k_collect, count, (const uint32_t*)d_ids.p, uint32_t(count), uint32_t(n),
+           (const uint32_t*)inv, (const float*)feat, (const float4*)pos, (const float4*)vel, (const float4*)aux[pp],
+           T.X + first * MLP_IN, T.pend + first * 3);
+    sync();
+    T.rows += count;
+    T.unfinished += count;
+    return first;
+  }
+
+  // second half, `win` seconds later: targets (a_eff - a_last) / |g|
+  void finish(Trainer& T, size_t first, const uint32_t* ids, size_t count, float win) {
+    if (T.dev != dev) throw Error(UBP_EINVAL, "trainer and engine on different devices");
+    if (first + count > T.rows || count > T.unfinished) throw Error(UBP_EINVAL, "no such pending rows");
+    if (!(win > 0.0f)) throw Error(UBP_EINVAL, "window must be positive");
+    if (count == 0) return;
+    launch(PASS_ELEM, k_inverse, n, (const uint32_t*)orig[pp], uint32_t(n), inv);
+    DevArr<uint32_t> d_ids(count);
+    HIPCHK(hipMemcpyAsync(d_ids.p, ids, count * 4, hipMemcpyHostToDevice, st));
+    launch(PASS_ELEM, k_target, count, (const uint32_t*)d_ids.p, uint32_t(count), uint32_t(n), (const uint32_t*)inv,
+           (const float4*)pos, (const float4*)(T.pend + first * 3), win, 1.0f / gmag(), T.T + first * 4);
+    sync();
+    T.unfinished -= count;
+  }
+
+  // features of the current state in original order (out: n x 32)
+  void features_host(float tau, float* out) {
+    if (!sur_on || needs_init) throw Error(UBP_EINVAL, "features need a marked, stepped engine");
+    sur_features(tau, nullptr);
+    std::vector<float> hf(32 * n);
+    std::vector<uint32_t> ho(n);
+    HIPCHK(hipMemcpyAsync(hf.data(), feat, hf.size() * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(ho.data(), orig[pp], n * 4, hipMemcpyDeviceToHost, st));
+    sync();
+    for (size_t s = 0; s < n; ++s) std::memcpy(out + size_t(ho[s]) * 32, hf.data() + s * 32, 128);
+  }
+
   void report_tuning(ubp_tuning* out, double step_ms) const {
     if (!out) return;
     for (int k = 0; k < UBP_NPASS; ++k) out->block[k] = pblock[k];
@@ -1044,6 +1322,9 @@ using ubp::Error;
 
 struct ubp_engine {
   Engine eng;
+};
+struct ubp_trainer {
+  ubp::Trainer tr;
 };
 
 #define UBP_API_BEGIN                 \
@@ -1382,6 +1663,137 @@ int ubp_set_tuning_ex(ubp_engine* e, const ubp_tuning* t) {
   if (!t) throw Error(UBP_EINVAL, "null tuning");
   e->eng.set_pass_blocks(t->block);
   e->eng.set_capacity(t->nbr_capacity);
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_surrogate_set_model(ubp_engine* e, const ubp_mlp_model* m) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  if (!m) throw Error(UBP_EINVAL, "null model");
+  if (e->eng.multi) throw Error(UBP_EINVAL, "the surrogate runs on one-scene engines");
+  e->eng.sur_set_model(m);
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_surrogate_run(ubp_engine* e, int ncycles, int K, float dt, float factor, float threshold,
+                      int verify_every, ubp_surrogate_stats* stats) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  if (!stats) throw Error(UBP_EINVAL, "null stats");
+  if (e->eng.n == 0) { std::memset(stats, 0, sizeof(*stats)); return UBP_OK; }
+  e->eng.sur_run(ncycles, K, dt, factor, threshold, verify_every, stats);
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_surrogate_mark(ubp_engine* e) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  e->eng.sur_mark();
+  e->eng.sync();
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_surrogate_step(ubp_engine* e, float dt, float tau) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  Engine& g = e->eng;
+  if (!g.sur_on || g.needs_init) throw Error(UBP_EINVAL, "mark a stepped state first");
+  if (!(dt > 0.0f)) throw Error(UBP_EINVAL, "dt must be positive");
+  hipLaunchKernelGGL(ubp::k_set_dt, dim3(1), dim3(64), 0, g.st, g.dts, dt, 1u);
+  LAUNCHCHK("k_set_dt");
+  g.sur_step(tau);
+  g.dt_valid = false;
+  g.sync();
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_surrogate_features(ubp_engine* e, float tau, float* out) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  if (!out) throw Error(UBP_EINVAL, "null output");
+  e->eng.features_host(tau, out);
+  return UBP_OK;
+  UBP_API_END
+}
+
+ubp_trainer* ubp_trainer_create(size_t capacity_rows, uint64_t seed) {
+  ubp::g_last_error.clear();
+  ubp_trainer* t = nullptr;
+  try {
+    t = new ubp_trainer();
+    t->tr.create(capacity_rows, seed);
+    return t;
+  } catch (const Error& err) {
+    ubp::g_last_error = err.what();
+  } catch (const std::exception& err) {
+    ubp::g_last_error = err.what();
+  }
+  delete t;
+  return nullptr;
+}
+
+void ubp_trainer_destroy(ubp_trainer* t) {
+  if (!t) return;
+  try { delete t; } catch (...) {}
+}
+
+#define UBP_TRAINER_CHECK(t)                                                        \
+  if (!(t)) throw Error(UBP_EINVAL, "null trainer handle");                         \
+  { hipError_t _s = hipSetDevice((t)->tr.dev);                                      \
+    if (_s != hipSuccess) throw Error(UBP_EHIP, std::string("hipSetDevice: ") + hipGetErrorString(_s)); }
+
+int ubp_trainer_collect(ubp_trainer* t, ubp_engine* e, float tau, const uint32_t* ids, size_t count,
+                        size_t* first_row) {
+  UBP_API_BEGIN
+  UBP_TRAINER_CHECK(t);
+  UBP_HANDLE_CHECK(e);
+  if (count && !ids) throw Error(UBP_EINVAL, "null ids");
+  const size_t f = e->eng.collect(t->tr, tau, ids, count);
+  if (first_row) *first_row = f;
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_trainer_finish(ubp_trainer* t, ubp_engine* e, size_t first_row, const uint32_t* ids, size_t count,
+                       float window) {
+  UBP_API_BEGIN
+  UBP_TRAINER_CHECK(t);
+  UBP_HANDLE_CHECK(e);
+  if (count && !ids) throw Error(UBP_EINVAL, "null ids");
+  e->eng.finish(t->tr, first_row, ids, count, window);
+  return UBP_OK;
+  UBP_API_END
+}
+
+size_t ubp_trainer_rows(const ubp_trainer* t) { return t ? t->tr.rows : 0; }
+
+int ubp_trainer_init_model(ubp_trainer* t, const ubp_mlp_model* init) {
+  UBP_API_BEGIN
+  UBP_TRAINER_CHECK(t);
+  t->tr.init_model(init);
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_trainer_fit(ubp_trainer* t, int steps, uint32_t batch, float lr, float lr_end, float* losses) {
+  UBP_API_BEGIN
+  UBP_TRAINER_CHECK(t);
+  t->tr.fit(steps, batch, lr, lr_end, losses);
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_trainer_get_model(ubp_trainer* t, ubp_mlp_model* out) {
+  UBP_API_BEGIN
+  UBP_TRAINER_CHECK(t);
+  if (!out) throw Error(UBP_EINVAL, "null model");
+  if (!t->tr.stats_set) throw Error(UBP_EINVAL, "no model yet (init_model)");
+  t->tr.M.download(out, t->tr.st);
   return UBP_OK;
   UBP_API_END
 }

@@ -163,7 +163,9 @@ __global__ UBP_LB void k_rank_gather(DevParams P0, Scenes S, const uint32_t* __r
                                      float4* __restrict__ pos, float4* __restrict__ vel,
                                      float4* __restrict__ dr, float4* __restrict__ wall,
                                      uint8_t* __restrict__ ft_dst, uint32_t* __restrict__ orig_dst,
-                                     const uint16_t* __restrict__ sid_src, uint16_t* __restrict__ sid_dst) {
+                                     const uint16_t* __restrict__ sid_src, uint16_t* __restrict__ sid_dst,
+                                     const float4* __restrict__ aux_src, float4* __restrict__ aux_dst,
+                                     float4* __restrict__ sr) {
   uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= P0.n) return;
   uint32_t s = stage_src[d];
@@ -189,6 +191,14 @@ __global__ UBP_LB void k_rank_gather(DevParams P0, Scenes S, const uint32_t* __r
   wall[t] = wall_t[s];
   ft_dst[t] = ft_src[s];
   orig_dst[t] = me;
+  if (aux_src) {  // surrogate mode: last-correction state + feature record
+    const float4 a = aux_src[s];
+    const float4 v = vel_t[s];
+    aux_dst[t] = a;
+    sr[3u * t] = p;
+    sr[3u * t + 1u] = make_float4(v.x, v.y, v.z, a.w);
+    sr[3u * t + 2u] = a;
+  }
 }
 
 // Neighbour structure, built in two passes each step:
@@ -863,6 +873,259 @@ __global__ UBP_LB void k_cfl_finalize(DevParams P, const uint32_t* __restrict__ 
     if (dt > 0.01f) dt = 0.01f;
     dtp[0] = dt;
   }
+}
+
+// ---- learned surrogate ----
+// Input features of particle i, pooled over its radius-2h neighbours in the
+// reference order (plain FP32 sums), from the feature record
+// sr = (x y z m)(v, delta_last)(a_last, .) of every slot:
+//   s0 = sum V_j W_ij + V_i W(0) (+ wall Omega)        V_j = m_j / rho0
+//   s1..3 = sum V_j grad_i W_ij          s4..6 = sum V_j W_ij (v_j - v_i)
+//   s7 = sum V_j (v_j - v_i) . grad W    s8..10 = sum V_j (delta_j - delta_i) grad W
+//   s11..13 = sum V_j W_ij (a_j - a_i)   (a, delta: state at the last correction)
+// scaled to O(1) by h, 1/U (U = c_s / 10) and 1/|g|; also v_i, a_last_i,
+// delta_last_i, the density change since the last correction, the SDF wall
+// distance / normal and the phase tau within the correction cycle.
+// rho_out (optional) receives the summation density rho0 s0.
+__global__ UBP_LB void k_feat(DevParams P, const float4* __restrict__ sr, const float4* __restrict__ wall,
+                              const uint8_t* __restrict__ ft, const uint32_t* __restrict__ start,
+                              const uint32_t* __restrict__ nbr, const uint32_t* __restrict__ nn, float tau,
+                              float inv_u, float inv_g, float* __restrict__ F, float* __restrict__ rho_out) {
+  uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
+  uint32_t i = blk * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const float4 pi = sr[3u * i];
+  const float4 vi = sr[3u * i + 1u];
+  const float4 ai = sr[3u * i + 2u];
+  const int f = ft[i];
+  const float ir0 = 1.0f / P.rho0[f];
+  float s0 = pi.w * ir0 * P.norm_h3;
+  float s1 = 0.0f, s2 = 0.0f, s3 = 0.0f, s4 = 0.0f, s5 = 0.0f, s6 = 0.0f, s7 = 0.0f;
+  float s8 = 0.0f, s9 = 0.0f, s10 = 0.0f, s11 = 0.0f, s12 = 0.0f, s13 = 0.0f;
+  for_neighbors<3>(P, i, pi, sr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
+    const float dx = pi.x - q.x;
+    const float dy = pi.y - q.y;
+    const float dz = pi.z - q.z;
+    const float dist_sq = dx * dx + dy * dy + dz * dz;
+    const float r = sqrtf(dist_sq);
+    const float qq = div_h(r, P);
+    const float w = wendland_q(qq, P);
+    float gx, gy, gz;
+    wgrad_q(dx, dy, dz, r, qq, P, gx, gy, gz);
+    const float4 vj = rj[1];
+    const float4 aj = rj[2];
+    const float vol = q.w * ir0;
+    const float vw = vol * w;
+    s0 += vw;
+    s1 += vol * gx;
+    s2 += vol * gy;
+    s3 += vol * gz;
+    const float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
+    s4 += vw * dvx;
+    s5 += vw * dvy;
+    s6 += vw * dvz;
+    s7 += vol * (dvx * gx + dvy * gy + dvz * gz);
+    const float dd = vj.w - vi.w;
+    s8 += vol * dd * gx;
+    s9 += vol * dd * gy;
+    s10 += vol * dd * gz;
+    s11 += vw * (aj.x - ai.x);
+    s12 += vw * (aj.y - ai.y);
+    s13 += vw * (aj.z - ai.z);
+  });
+  const float4 wl = wall[i];
+  float sw = 2.0f, wx = 0.0f, wy = 0.0f, wz = 0.0f;
+  if (wl.w != __builtin_inff()) {
+    const float s = wl.w / P.h;
+    if (s < 2.0f) {
+      s0 += wall_omega(s);
+      sw = s < 0.0f ? 0.0f : s;
+      const float sc = 1.0f - 0.5f * sw;
+      wx = wl.x * sc;
+      wy = wl.y * sc;
+      wz = wl.z * sc;
+    }
+  }
+  if (rho_out) rho_out[i] = P.rho0[f] * s0;
+  const float h = P.h;
+  const float d0 = s0 - 1.0f;
+  float4* o = reinterpret_cast<float4*>(F + size_t(i) * 32u);
+  o[0] = make_float4(d0, s1 * h, s2 * h, s3 * h);
+  o[1] = make_float4(s4 * inv_u, s5 * inv_u, s6 * inv_u, s7 * h * inv_u);
+  o[2] = make_float4(s8 * h * 100.0f, s9 * h * 100.0f, s10 * h * 100.0f, s11 * inv_g);
+  o[3] = make_float4(s12 * inv_g, s13 * inv_g, vi.x * inv_u, vi.y * inv_u);
+  o[4] = make_float4(vi.z * inv_u, ai.x * inv_g, ai.y * inv_g, ai.z * inv_g);
+  o[5] = make_float4(vi.w * 100.0f, (d0 - vi.w) * 100.0f, sw, wx);
+  o[6] = make_float4(wy, wz, tau, 0.0f);
+  o[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// Correction-cycle start: a_last = a, delta_last = rho / rho0 - 1 (slot
+// order, into aux and the feature record); x0[orig] = x for the judge.
+__global__ UBP_LB void k_aux_set(DevParams P, const float4* __restrict__ acc, const float* __restrict__ rho,
+                                 const float4* __restrict__ pos, const uint8_t* __restrict__ ft,
+                                 const uint32_t* __restrict__ orig, float4* __restrict__ aux,
+                                 float4* __restrict__ sr, float4* __restrict__ x0) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.n) return;
+  const float4 a = acc[i];
+  const float dl = rho[i] / P.rho0[ft[i]] - 1.0f;
+  const float4 v = make_float4(a.x, a.y, a.z, dl);
+  aux[i] = v;
+  sr[3u * i + 1u].w = dl;
+  sr[3u * i + 2u] = v;
+  x0[orig[i]] = pos[i];
+}
+
+// Judge of a correction cycle: per-block FP64 sums of |a_pred - a|^2,
+// |a - a_last|^2 and |x - x0|^2 (the plain solver's displacement over the
+// cycle).  Written for 256 threads.
+__global__ __launch_bounds__(256) void k_judge(DevParams P, const float4* __restrict__ acc,
+                                               const float4* __restrict__ apred, const float4* __restrict__ aux,
+                                               const float4* __restrict__ pos, const uint32_t* __restrict__ orig,
+                                               const float4* __restrict__ x0, double* __restrict__ part) {
+  __shared__ double sm[3][256];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0.0, hld = 0.0, dsp = 0.0;
+  if (i < P.n) {
+    const float4 a = acc[i], b = apred[i], c = aux[i], x = pos[i], y = x0[orig[i]];
+    const double ex = double(b.x) - a.x, ey = double(b.y) - a.y, ez = double(b.z) - a.z;
+    const double hx = double(c.x) - a.x, hy = double(c.y) - a.y, hz = double(c.z) - a.z;
+    const double dx = double(x.x) - y.x, dy = double(x.y) - y.y, dz = double(x.z) - y.z;
+    e = ex * ex + ey * ey + ez * ez;
+    hld = hx * hx + hy * hy + hz * hz;
+    dsp = dx * dx + dy * dy + dz * dz;
+  }
+  sm[0][threadIdx.x] = e;
+  sm[1][threadIdx.x] = hld;
+  sm[2][threadIdx.x] = dsp;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (int(threadIdx.x) < s)
+      for (int k = 0; k < 3; ++k) sm[k][threadIdx.x] += sm[k][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) part[3 * blockIdx.x + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+__global__ UBP_LB void k_scale_dt(float* __restrict__ dst, const float* __restrict__ src, float factor) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) dst[0] = src[0] * factor;
+}
+
+// inv[orig[s]] = s
+__global__ UBP_LB void k_inverse(const uint32_t* __restrict__ orig, uint32_t n, uint32_t* __restrict__ inv) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) inv[orig[s]] = s;
+}
+
+// Training rows for particles ids[k] (original indices), first half: the
+// features of the current state and, pending the target, (x, v, a_last).
+__global__ UBP_LB void k_collect(const uint32_t* __restrict__ ids, uint32_t count, uint32_t n,
+                                 const uint32_t* __restrict__ inv, const float* __restrict__ F,
+                                 const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                 const float4* __restrict__ aux, float* __restrict__ X, float4* __restrict__ pend) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const uint32_t id = ids[k];
+  if (id >= n) return;
+  const uint32_t s = inv[id];
+  const float4* src = reinterpret_cast<const float4*>(F + size_t(s) * 32u);
+  float4* dst = reinterpret_cast<float4*>(X + size_t(k) * 32u);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) dst[c] = src[c];
+  const float4 x = pos[s], v = vel[s], a = aux[s];
+  pend[3u * k] = make_float4(x.x, x.y, x.z, 0.0f);
+  pend[3u * k + 1u] = make_float4(v.x, v.y, v.z, 0.0f);
+  pend[3u * k + 2u] = make_float4(a.x, a.y, a.z, 0.0f);
+}
+
+// Second half, a window `win` later: the effective acceleration over the
+// window, a_eff = 2 (x(t + win) - x(t) - win v(t)) / win^2 (the constant
+// acceleration that moves the particle from x(t) with v(t) to x(t + win)),
+// and the target (a_eff - a_last) / |g|.
+__global__ UBP_LB void k_target(const uint32_t* __restrict__ ids, uint32_t count, uint32_t n,
+                                const uint32_t* __restrict__ inv, const float4* __restrict__ pos,
+                                const float4* __restrict__ pend, float win, float inv_g, float* __restrict__ T) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const uint32_t id = ids[k];
+  if (id >= n) return;
+  const float4 x1 = pos[inv[id]];
+  const float4 x0 = pend[3u * k], v0 = pend[3u * k + 1u], al = pend[3u * k + 2u];
+  const float c = 2.0f / (win * win);
+  const float ax = c * ((x1.x - x0.x) - win * v0.x);
+  const float ay = c * ((x1.y - x0.y) - win * v0.y);
+  const float az = c * ((x1.z - x0.z) - win * v0.z);
+  reinterpret_cast<float4*>(T)[k] = make_float4((ax - al.x) * inv_g, (ay - al.y) * inv_g, (az - al.z) * inv_g, 0.0f);
+}
+
+// Sums of |pos - ref[orig]| and |pos - x0[orig]| (per-block FP64, 256 threads):
+// the position error of a state against a reference, and the reference's
+// displacement since x0.
+__global__ __launch_bounds__(256) void k_poserr(uint32_t n, const float4* __restrict__ pos,
+                                                const uint32_t* __restrict__ orig, const float4* __restrict__ ref,
+                                                const float4* __restrict__ x0, double* __restrict__ part) {
+  __shared__ double sm[2][256];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  double e = 0.0, d = 0.0;
+  if (i < n) {
+    const float4 x = pos[i], r = ref[orig[i]], y = x0[orig[i]];
+    const double ex = double(r.x) - x.x, ey = double(r.y) - x.y, ez = double(r.z) - x.z;
+    const double dx = double(x.x) - y.x, dy = double(x.y) - y.y, dz = double(x.z) - y.z;
+    e = sqrt(ex * ex + ey * ey + ez * ez);
+    d = sqrt(dx * dx + dy * dy + dz * dz);
+  }
+  sm[0][threadIdx.x] = e;
+  sm[1][threadIdx.x] = d;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (int(threadIdx.x) < s) { sm[0][threadIdx.x] += sm[0][threadIdx.x + s]; sm[1][threadIdx.x] += sm[1][threadIdx.x + s]; }
+    __syncthreads();
+  }
+  if (threadIdx.x < 2) part[2 * blockIdx.x + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+// ref[orig[i]] = pos[i]
+__global__ UBP_LB void k_store_pos(uint32_t n, const float4* __restrict__ pos, const uint32_t* __restrict__ orig,
+                                   float4* __restrict__ ref) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) ref[orig[i]] = pos[i];
+}
+
+// Per-block FP64 sums and sums of squares of the columns of a row-major
+// [rows][ncol] FP32 matrix (ncol <= 64), written for 256 threads.
+__global__ __launch_bounds__(256) void k_colstats(const float* __restrict__ A, uint32_t rows, uint32_t ncol,
+                                                  uint32_t stride, double* __restrict__ part) {
+  __shared__ double ps[256], pq[256];
+  const uint32_t c = threadIdx.x % ncol;
+  const uint32_t per = 256u / ncol;  // threads per column
+  double s = 0.0, q = 0.0;
+  if (threadIdx.x < per * ncol) {
+    for (uint32_t r = blockIdx.x * per + threadIdx.x / ncol; r < rows; r += gridDim.x * per) {
+      const double v = A[size_t(r) * stride + c];
+      s += v;
+      q += v * v;
+    }
+  }
+  ps[threadIdx.x] = s;
+  pq[threadIdx.x] = q;
+  __syncthreads();
+  if (threadIdx.x < ncol) {  // fixed order: deterministic
+    double a = 0.0, b = 0.0;
+    for (uint32_t k = 0; k < per; ++k) { a += ps[k * ncol + threadIdx.x]; b += pq[k * ncol + threadIdx.x]; }
+    part[size_t(blockIdx.x) * 2u * ncol + threadIdx.x] = a;
+    part[size_t(blockIdx.x) * 2u * ncol + ncol + threadIdx.x] = b;
+  }
+}
+
+// Tn = (T - mean) / std for the three target columns
+__global__ UBP_LB void k_norm_targets(const float* __restrict__ T, uint32_t rows, const float* __restrict__ mean,
+                                      const float* __restrict__ sd, float* __restrict__ Tn) {
+  const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  const float4 t = reinterpret_cast<const float4*>(T)[r];
+  reinterpret_cast<float4*>(Tn)[r] =
+      make_float4((t.x - mean[0]) / sd[0], (t.y - mean[1]) / sd[1], (t.z - mean[2]) / sd[2], 0.0f);
 }
 
 // Per-scene dt (many-scene mode): the 64-thread workgroup s reduces the

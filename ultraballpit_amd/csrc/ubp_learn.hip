@@ -100,6 +100,8 @@ struct Trainer {
   float* X = nullptr;   // [cap][32] raw features
   float* T = nullptr;   // [cap][4] raw targets (3 used), normalised at fit time
   float* Tn = nullptr;  // normalised targets
+  float4* pend = nullptr;  // [cap][3] (x, v, a_last) of rows awaiting their target
+  size_t unfinished = 0;
   MlpBuf M;
   float *m1 = nullptr, *m2 = nullptr, *grad = nullptr, *part = nullptr;
   uint32_t* idx = nullptr;
@@ -111,10 +113,125 @@ struct Trainer {
   ~Trainer() { release(); }
   void release() {
     if (st) (void)hipStreamSynchronize(st);
-    dfree(X); dfree(T); dfree(Tn); dfree(m1); dfree(m2); dfree(grad); dfree(part); dfree(idx);
+    dfree(X); dfree(T); dfree(Tn); dfree(pend); dfree(m1); dfree(m2); dfree(grad); dfree(part); dfree(idx);
     M.release();
     if (st) { (void)hipStreamDestroy(st); st = nullptr; }
   }
+  static uint64_t splitmix(uint64_t& s) {
+    uint64_t z = (s += 0x9e3779b97f4a7c15ull);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+  }
+
+  // column means / standard deviations of a [rows][stride] matrix
+  void colstats(const float* A, uint32_t ncol, uint32_t stride, std::vector<double>& mean,
+                std::vector<double>& sd) {
+    const uint32_t g = uint32_t(std::min<size_t>(1024, (rows + 255) / 256 + 1));
+    DevArr<double> part(size_t(g) * 2 * ncol);
+    hipLaunchKernelGGL(k_colstats, dim3(g), dim3(256), 0, st, A, uint32_t(rows), ncol, stride, part.p);
+    LAUNCHCHK("k_colstats");
+    std::vector<double> h(size_t(g) * 2 * ncol);
+    HIPCHK(hipMemcpyAsync(h.data(), part.p, h.size() * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    mean.assign(ncol, 0.0);
+    sd.assign(ncol, 0.0);
+    std::vector<double> q(ncol, 0.0);
+    for (uint32_t b = 0; b < g; ++b)
+      for (uint32_t c = 0; c < ncol; ++c) { mean[c] += h[size_t(b) * 2 * ncol + c]; q[c] += h[size_t(b) * 2 * ncol + ncol + c]; }
+    for (uint32_t c = 0; c < ncol; ++c) {
+      mean[c] /= double(rows);
+      sd[c] = std::sqrt(std::max(0.0, q[c] / double(rows) - mean[c] * mean[c]));
+    }
+  }
+
+  // Model to start from: `init` (parameters and normalisation kept), or He
+  // initialisation from the seed with normalisation from the data set.
+  void init_model(const ubp_mlp_model* init) {
+    if (rows == 0) throw Error(UBP_EINVAL, "the trainer holds no rows");
+    if (unfinished) throw Error(UBP_EINVAL, "rows without targets remain (ubp_trainer_finish)");
+    std::vector<float> hm(sizeof(ubp_mlp_model) / 4, 0.0f);
+    ubp_mlp_model* m = reinterpret_cast<ubp_mlp_model*>(hm.data());
+    if (init) {
+      *m = *init;
+    } else {
+      uint64_t s = seed;
+      auto normal = [&]() {
+        const double u1 = (double(splitmix(s) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        const double u2 = double(splitmix(s) >> 11) * (1.0 / 9007199254740992.0);
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+      };
+      auto fill = [&](float* w, int fan_in, int count, double gain) {
+        const double sdv = gain * std::sqrt(2.0 / fan_in);
+        for (int k = 0; k < count; ++k) w[k] = float(sdv * normal());
+      };
+      fill(m->w1, MLP_IN, MLP_IN * MLP_HID, 1.0);
+      fill(m->w2, MLP_HID, MLP_HID * MLP_HID, 1.0);
+      fill(m->w3, MLP_HID, MLP_HID * MLP_HID, 1.0);
+      fill(m->w4, MLP_HID, MLP_HID * MLP_OUT, 0.1);  // small initial output
+      std::vector<double> mu, sd;
+      colstats(X, MLP_IN, MLP_IN, mu, sd);
+      for (int c = 0; c < MLP_IN; ++c) {
+        m->in_mean[c] = float(mu[c]);
+        m->in_inv_std[c] = sd[c] > 1e-12 ? float(1.0 / sd[c]) : 0.0f;
+      }
+      colstats(T, MLP_OUT, 4, mu, sd);
+      for (int c = 0; c < MLP_OUT; ++c) {
+        m->out_mean[c] = float(mu[c]);
+        m->out_std[c] = sd[c] > 1e-12 ? float(sd[c]) : 1.0f;
+      }
+    }
+    M.upload(m, st);
+    hipLaunchKernelGGL(k_norm_targets, dim3(uint32_t((rows + 255) / 256)), dim3(256), 0, st, (const float*)T,
+                       uint32_t(rows), (const float*)(M.stat + 2 * MLP_IN), (const float*)(M.stat + 2 * MLP_IN + MLP_OUT),
+                       Tn);
+    LAUNCHCHK("k_norm_targets");
+    HIPCHK(hipMemsetAsync(m1, 0, MLP_NPARAM * 4, st));
+    HIPCHK(hipMemsetAsync(m2, 0, MLP_NPARAM * 4, st));
+    HIPCHK(hipStreamSynchronize(st));
+    step = 0;
+    stats_set = true;
+  }
+
+  // Adam over `steps` mini-batches of `batch` rows drawn with replacement;
+  // learning rate cosine-decayed from lr to lr_end.  losses[s] = the batch
+  // loss (normalised targets) before step s.
+  void fit(int steps, uint32_t batch, float lr, float lr_end, float* losses) {
+    if (!stats_set) throw Error(UBP_EINVAL, "call init_model first");
+    if (steps < 0 || batch == 0 || batch > (1u << 24)) throw Error(UBP_EINVAL, "bad fit parameters");
+    const uint32_t nblk = (batch + TR - 1) / TR;
+    if (nblk > part_blk) {
+      dfree(part);
+      dalloc(&part, size_t(nblk) * (MLP_NPARAM + 1));
+      part_blk = nblk;
+    }
+    if (batch > idx_cap) {
+      dfree(idx);
+      dalloc(&idx, batch);
+      idx_cap = batch;
+    }
+    DevArr<float> dl(std::max(1, steps));
+    for (int s = 0; s < steps; ++s) {
+      hipLaunchKernelGGL(k_mlp_sample, dim3((batch + 255) / 256), dim3(256), 0, st, idx, batch, uint32_t(rows),
+                         seed, step);
+      LAUNCHCHK("k_mlp_sample");
+      mlp_grad(M, X, Tn, idx, batch, part, grad, st);
+      ++step;
+      const double c1 = 1.0 - std::pow(0.9, double(step)), c2 = 1.0 - std::pow(0.999, double(step));
+      const double ph = steps > 1 ? double(s) / double(steps - 1) : 1.0;
+      const float lrs = float(lr_end + 0.5 * (lr - lr_end) * (1.0 + std::cos(3.141592653589793 * ph)));
+      hipLaunchKernelGGL(k_mlp_adam, dim3((MLP_NPARAM + 255) / 256), dim3(256), 0, st, M.prm, m1, m2,
+                         (const float*)grad, lrs, 0.9f, 0.999f, 1e-8f, float(c1), float(c2), M.frag);
+      LAUNCHCHK("k_mlp_adam");
+      HIPCHK(hipMemcpyAsync(dl.p + s, grad + MLP_NPARAM, 4, hipMemcpyDeviceToDevice, st));
+    }
+    std::vector<float> h(std::max(1, steps));
+    HIPCHK(hipMemcpyAsync(h.data(), dl.p, h.size() * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    if (losses)
+      for (int s = 0; s < steps; ++s) losses[s] = float(double(h[s]) / (3.0 * double(batch)));
+  }
+
   void create(size_t capacity, uint64_t sd) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) throw Error(UBP_ENODEV, "no HIP device visible");
@@ -127,6 +244,7 @@ struct Trainer {
     dalloc(&X, cap * MLP_IN);
     dalloc(&T, cap * 4);
     dalloc(&Tn, cap * 4);
+    dalloc(&pend, cap * 3);
     M.alloc();
     dalloc(&m1, MLP_NPARAM);
     dalloc(&m2, MLP_NPARAM);
