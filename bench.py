@@ -144,7 +144,7 @@ def surrogate_bench(cycles: int, warm_steps: int, model_path: str):
     sc = scenes.dam_break()
     dt = surrogate.fixed_dt(sc)
     net = surrogate.evaluate(sc, model, cycles=cycles, dt=dt, warm_steps=warm_steps)
-    held = surrogate.evaluate(sc, surrogate.zero_model(model), cycles=cycles, dt=dt, warm_steps=warm_steps)
+    held = surrogate.evaluate(sc, None, cycles=cycles, dt=dt, warm_steps=warm_steps)
     judged = surrogate.evaluate(sc, model, cycles=cycles, dt=dt, warm_steps=warm_steps, verify_every=4,
                                 threshold=0.01)
     # discretisation floor: plain at dt/2 against plain at dt
@@ -171,11 +171,12 @@ def surrogate_bench(cycles: int, warm_steps: int, model_path: str):
             "cycles": cycles, "warm_steps": warm_steps, "dt": dt, "sim_time_s": net["sim_time"],
             "plain_s": net["plain_s"], "surrogate_s": net["surrogate_s"], "speedup": net["speedup"],
             "rel_pos_err": net["rel_err"], "meets_1pct": bool(net["rel_err"] is not None and net["rel_err"] <= 0.01),
+            "guard_trips": net["stats"]["guard_trips"],
             "held_accel_rel_pos_err": held["rel_err"], "plain_dt_half_rel_diff": floor,
-            "judged_every_4": {"speedup": judged["speedup"], "rel_pos_err": judged["rel_err"],
+            "judged_every_4": {"speedup": judged["speedup"], "rel_pos_err": judged["rel_err"], "guard_trips": judged["stats"]["guard_trips"],
                                "fallbacks": judged["stats"]["fallbacks"], "verified": judged["stats"]["verified"],
                                "cycle_err_mean": judged["stats"]["err_mean"], "cycle_err_max": judged["stats"]["err_max"]},
-            "training": {k: train_info.get(k) for k in ("rollouts", "particles", "rows", "steps", "batch",
+            "training": {k: train_info.get(k) for k in ("rollouts", "particles", "rows", "rounds", "steps", "batch",
                                                         "loss_first", "loss_last", "wall_s")}}
 
 

@@ -231,16 +231,19 @@ int ubp_get_tuning(const ubp_engine *e, ubp_tuning *out);
 int ubp_set_tuning_ex(ubp_engine *e, const ubp_tuning *t);
 
 /* ---- Learned surrogate (no reference counterpart).
- * A per-particle network predicts the acceleration from features pooled over
+ * A per-particle network predicts accelerations from features pooled over
  * the radius-2h neighbour lists (kernel-weighted sums of relative positions,
  * velocities, density and last-corrected accelerations; SDF wall distance):
  *   y = W4 relu(W3 relu(W2 relu(W1 x + b1) + b2) + b3) + b4,
- *   x = bf16((f - in_mean) * in_inv_std),   a = a_last + |g| (out_mean + out_std y).
+ *   x = bf16((f - in_mean) * in_inv_std),   a = |g| (out_mean + out_std y),
+ * six outputs: the effective accelerations of a network step of length w for
+ * the position (a[0..2]: x += w v + w^2 a_x / 2) and the velocity
+ * (a[3..5]: v += w a_v).
  * Weights are rounded to BF16 for the MFMA products (FP32 accumulate);
  * activations are rounded to BF16 between layers.  Host structs hold FP32. */
 #define UBP_MLP_IN 32
 #define UBP_MLP_HID 128
-#define UBP_MLP_OUT 3
+#define UBP_MLP_OUT 6
 typedef struct {
   float w1[UBP_MLP_IN * UBP_MLP_HID];   /* [in][hid], row-major */
   float b1[UBP_MLP_HID];
@@ -257,47 +260,54 @@ typedef struct {
 /* Number of trained parameters (w1 .. b4, contiguous at the start). */
 #define UBP_MLP_NPARAM (UBP_MLP_IN * UBP_MLP_HID + UBP_MLP_HID + 2 * (UBP_MLP_HID * UBP_MLP_HID + UBP_MLP_HID) + UBP_MLP_HID * UBP_MLP_OUT + UBP_MLP_OUT)
 
-/* Network output y (normalised, rows x 3) for raw features x (rows x 32),
+/* Network output y (normalised, rows x 6) for raw features x (rows x 32),
  * host buffers, through the engine's BF16-MFMA inference kernel on the
  * current device.  Synchronous. */
 int ubp_mlp_forward(const ubp_mlp_model *m, const float *x, size_t rows, float *y);
-/* Loss L = sum over rows and outputs of (y - t)^2 / (3 rows) for normalised
- * targets t (rows x 3), and dL/dparams through the training kernels
+/* Loss L = sum over rows and outputs of (y - t)^2 / (6 rows) for normalised
+ * targets t (rows x 6), and dL/dparams through the training kernels
  * (fused BF16-MFMA forward + backward per 128-row tile, deterministic
  * reduction).  grad uses the parameter fields of ubp_mlp_model.  Synchronous. */
 int ubp_mlp_loss_grad(const ubp_mlp_model *m, const float *x, const float *t, size_t rows,
                       ubp_mlp_model *grad, double *loss);
 /* Surrogate rollout on an engine (one-scene handles).  Cycle: mark the
  * current plain state (a_last, delta_last), K network steps of factor * dt
- * (the network predicts the effective acceleration over its step), one plain
- * step (the correction).  Every cycle records an acceleration-based error
+ * (x += w v + w^2 a_x / 2, v += w a_v with the network's a_x, a_v at the
+ * current state), one plain step (the correction).  Every cycle records an acceleration-based error
  * estimate est = 0.5 rms|a_net - a_plain| T_cycle^2 / rms|x - x_start|.
  * Every `verify_every`-th cycle (0: never) is the judge: the cycle is rerun
  * with plain WCSPH from its start, the measured position error
  *   err = mean|x_net - x_plain| / mean|x_plain - x_start|
  * is recorded, the plain state is kept, and err > threshold falls back to
- * plain cycles until the next verification.  dt > 0 is a fixed plain dt,
- * dt <= 0 the device CFL dt.  Synchronous. */
+ * plain cycles until the next verification.  Every cycle is also guarded: if
+ * the corrected state's max |rho/rho0 - 1| exceeds 4x the plain solver's at
+ * the start of the run (at least 0.05), the cycle is rewound (device
+ * snapshot) and rerun plain.  dt > 0 is a fixed plain dt, dt <= 0 the device
+ * CFL dt.  Synchronous. */
 typedef struct {
-  uint32_t cycles, plain_steps, surrogate_steps, fallbacks, verified;
+  uint32_t cycles, plain_steps, surrogate_steps, fallbacks, verified, guard_trips;
   double sim_time;                     /* simulated seconds advanced */
+  double guard;                        /* density-deviation guard used */
   double err_last, err_max, err_mean;  /* measured error of verified cycles */
   double est_mean, held_est_mean;      /* estimate: network, and a = a_last (no network) */
 } ubp_surrogate_stats;
+/* m == NULL selects the network-free baseline (both step accelerations =
+ * a_last, the held acceleration of the last correction). */
 int ubp_surrogate_set_model(ubp_engine *e, const ubp_mlp_model *m);
 int ubp_surrogate_run(ubp_engine *e, int ncycles, int K, float dt, float factor, float threshold,
                       int verify_every, ubp_surrogate_stats *stats);
 /* Building blocks: start a cycle at the current (stepped) state; one network
- * step of size dt at cycle phase tau; the features of the current state in
- * original order (out: n x 32; needs a step since the first mark). */
+ * step of size dt landing at cycle phase tau; the features of the current
+ * state in original order (out: n x 32; needs a step since the first mark). */
 int ubp_surrogate_mark(ubp_engine *e);
 int ubp_surrogate_step(ubp_engine *e, float dt, float tau);
 int ubp_surrogate_features(ubp_engine *e, float tau, float *out);
 
 /* Trainer: a device data set of (features, target) rows collected from
- * engines, and Adam on the training kernels.  The target of a row is
- * (a_eff - a_last)/|g|, a_eff = 2 (x(t + w) - x(t) - w v(t)) / w^2 being the
- * effective acceleration over a window w of the plain trajectory. */
+ * engines, and Adam on the training kernels.  The targets of a row are
+ * (a_x, a_v)/|g| with the effective accelerations over a
+ * window w of the plain trajectory a_x = 2 (x(t + w) - x(t) - w v(t)) / w^2
+ * and a_v = (v(t + w) - v(t)) / w. */
 typedef struct ubp_trainer ubp_trainer;
 ubp_trainer *ubp_trainer_create(size_t capacity_rows, uint64_t seed);
 void ubp_trainer_destroy(ubp_trainer *t);

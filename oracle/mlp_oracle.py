@@ -16,7 +16,7 @@ from __future__ import annotations
 
 import numpy as np
 
-IN, HID, OUT = 32, 128, 3
+IN, HID, OUT = 32, 128, 6
 PARAMS = (("w1", (IN, HID)), ("b1", (HID,)), ("w2", (HID, HID)), ("b2", (HID,)),
           ("w3", (HID, HID)), ("b3", (HID,)), ("w4", (HID, OUT)), ("b4", (OUT,)))
 STATS = (("in_mean", (IN,)), ("in_inv_std", (IN,)), ("out_mean", (OUT,)), ("out_std", (OUT,)))
@@ -68,7 +68,7 @@ def _norm_input(m, x, rounding):
 
 
 def forward(m: dict, x, rounding: bool = True, keep: bool = False):
-    """Network output y (rows x 3).  keep=True also returns the activations."""
+    """Network output y (rows x OUT).  keep=True also returns the activations."""
     r = bf16 if rounding else (lambda a: np.asarray(a, np.float64))
     W = [np.asarray(r(m[f"w{k}"]), np.float64) for k in (1, 2, 3, 4)]
     B = [np.asarray(m[f"b{k}"], np.float32 if rounding else np.float64) for k in (1, 2, 3, 4)]
@@ -89,18 +89,18 @@ def forward(m: dict, x, rounding: bool = True, keep: bool = False):
 
 
 def loss_grad(m: dict, x, t, rounding: bool = True):
-    """L = sum (y - t)^2 / (3 rows) and dL/dparams (dict, FP64)."""
+    """L = sum (y - t)^2 / (OUT rows) and dL/dparams (dict, FP64)."""
     rows = len(x)
     y, acts = forward(m, x, rounding, keep=True)
     t = np.asarray(t, np.float32 if rounding else np.float64)
     e = y - t
-    L = float(np.sum(np.asarray(e, np.float64) ** 2) / (3.0 * rows))
+    L = float(np.sum(np.asarray(e, np.float64) ** 2) / (float(OUT) * rows))
     if rounding:
-        scale = np.float32(2.0) / (np.float32(3.0) * np.float32(rows))
+        scale = np.float32(2.0) / (np.float32(OUT) * np.float32(rows))
         dz = bf16(e.astype(np.float32) * scale).astype(np.float64)
         W = [bf16(m[f"w{k}"]).astype(np.float64) for k in (1, 2, 3, 4)]
     else:
-        dz = e * (2.0 / (3.0 * rows))
+        dz = e * (2.0 / (float(OUT) * rows))
         W = [np.asarray(m[f"w{k}"], np.float64) for k in (1, 2, 3, 4)]
     g = {}
     for k in (4, 3, 2, 1):
@@ -114,10 +114,12 @@ def loss_grad(m: dict, x, t, rounding: bool = True):
     return L, g
 
 
-def predict_accel(m: dict, feats, a_last, gmag):
-    """Engine output: a = a_last + |g| (out_mean + out_std y)."""
+def predict_accel(m: dict, feats, gmag):
+    """Engine outputs (a_x, a_v): |g| (out_mean + out_std y) split into the
+    position and velocity updates."""
     y = forward(m, feats)
-    return np.asarray(a_last, np.float32) + np.float32(gmag) * (m["out_mean"] + m["out_std"] * y)
+    a = np.float32(gmag) * (m["out_mean"] + m["out_std"] * y)
+    return a[:, :3], a[:, 3:]
 
 
 def adam_step(m: dict, g: dict, state: dict, lr, beta1=0.9, beta2=0.999, eps=1e-8):

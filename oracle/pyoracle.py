@@ -214,7 +214,7 @@ class Oracle:
 
     def step_features(self, aux, tau: float, dt: float = 0.0) -> np.ndarray:
         """One step (dt <= 0: CFL dt) returning the surrogate features of the
-        state after its force pass (n x 32); aux: n x 4 (a_last, delta_last)."""
+        state it ends in (n x 32); aux: n x 4 (a_last, delta_last)."""
         a = np.ascontiguousarray(aux, np.float32).reshape(self.n, 4)
         out = np.zeros((self.n, 32), np.float32)
         lib().orc_step_features(self.h, float(dt), a.ctypes.data_as(C.POINTER(C.c_float)), float(tau),

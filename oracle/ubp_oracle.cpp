@@ -384,7 +384,7 @@ template <class R> struct Sim {
   State<R> ckpt;
   Grid<R> ckpt_grid;
   std::vector<uint32_t> last_nbr_count;
-  // surrogate feature hook: computed inside step() after the forces
+  // surrogate feature hook: computed at the end of step()
   const float* feat_aux = nullptr;
   float feat_tau = 0.0f;
   float* feat_out = nullptr;
@@ -674,12 +674,12 @@ template <class R> struct Sim {
       if (p.z[i] > dmax[2]) { p.z[i] = dmax[2]; if (p.vz[i] > R(0)) p.vz[i] = -rest * p.vz[i]; }
     }
     compute_forces(dt);
-    if (feat_out) features(feat_aux, R(feat_tau), feat_out);
     for (size_t i = 0; i < n; ++i) {
       p.vx[i] += p.ax[i] * half_dt;
       p.vy[i] += p.ay[i] * half_dt;
       p.vz[i] += p.az[i] * half_dt;
     }
+    if (feat_out) features(feat_aux, R(feat_tau), feat_out);
   }
 
   // Surrogate input features of the engine's feature pass
@@ -748,7 +748,7 @@ template <class R> struct Sim {
                        s12 * inv_g, s13 * inv_g, p.vx[i] * inv_u, p.vy[i] * inv_u,
                        p.vz[i] * inv_u, avx * inv_g, avy * inv_g, avz * inv_g,
                        dli * R(100), (d0 - dli) * R(100), sw, wx,
-                       wy, wz, tau, R(0), R(0), R(0), R(0), R(0)};
+                       wy, wz, tau, h * R(100), R(0), R(0), R(0), R(0)};
       for (int k = 0; k < 32; ++k) out[32 * i + k] = float(o[k]);
     }
   }
@@ -935,7 +935,7 @@ void orc_step_auto(void* h, int nsteps, double* dts) {
 }
 
 // One step (dt <= 0: dt = compute_timestep) that also writes the surrogate
-// features of the state after its force pass (out: n x 32; aux: n x 4).
+// features of the state it ends in (out: n x 32; aux: n x 4).
 void orc_step_features(void* h, double dt, const float* aux, float tau, float* out) {
   Handle* hd = static_cast<Handle*>(h);
   if (dt <= 0.0) dt = orc_timestep(h);

@@ -87,7 +87,7 @@ class _Tuning(C.Structure):
     _fields_ = [("block", C.c_uint32 * NPASS), ("nbr_capacity", C.c_uint32), ("step_ms", C.c_double)]
 
 
-MLP_IN, MLP_HID, MLP_OUT = 32, 128, 3
+MLP_IN, MLP_HID, MLP_OUT = 32, 128, 6
 MLP_FIELDS = (("w1", (MLP_IN, MLP_HID)), ("b1", (MLP_HID,)), ("w2", (MLP_HID, MLP_HID)), ("b2", (MLP_HID,)),
               ("w3", (MLP_HID, MLP_HID)), ("b3", (MLP_HID,)), ("w4", (MLP_HID, MLP_OUT)), ("b4", (MLP_OUT,)),
               ("in_mean", (MLP_IN,)), ("in_inv_std", (MLP_IN,)), ("out_mean", (MLP_OUT,)),
@@ -114,7 +114,8 @@ def _model_from_c(c: _MlpModel) -> dict:
 
 class _SurStats(C.Structure):
     _fields_ = [("cycles", C.c_uint32), ("plain_steps", C.c_uint32), ("surrogate_steps", C.c_uint32),
-                ("fallbacks", C.c_uint32), ("verified", C.c_uint32), ("sim_time", C.c_double),
+                ("fallbacks", C.c_uint32), ("verified", C.c_uint32), ("guard_trips", C.c_uint32),
+                ("sim_time", C.c_double), ("guard", C.c_double),
                 ("err_last", C.c_double), ("err_max", C.c_double), ("err_mean", C.c_double),
                 ("est_mean", C.c_double), ("held_est_mean", C.c_double)]
 
@@ -398,7 +399,11 @@ class HipKernel:
         return int(lib().ubp_scene_count(self._h))
 
     # -- learned surrogate ----------------------------------------------------
-    def surrogate_set_model(self, model: dict):
+    def surrogate_set_model(self, model: dict | None):
+        """model=None: the network-free baseline a = a_last (held acceleration)."""
+        if model is None:
+            _check(lib().ubp_surrogate_set_model(self._h, None), "surrogate_set_model")
+            return
         mc = _model_to_c(model)
         _check(lib().ubp_surrogate_set_model(self._h, C.byref(mc)), "surrogate_set_model")
 
@@ -638,7 +643,7 @@ def load_model(path: str) -> dict:
 
 
 def mlp_forward(model: dict, x) -> np.ndarray:
-    """Surrogate network output y (rows x 3, normalised) on the GPU (BF16 MFMA)."""
+    """Surrogate network output y (rows x 6, normalised) on the GPU (BF16 MFMA)."""
     x = np.ascontiguousarray(x, np.float32)
     rows = x.shape[0]
     y = np.zeros((rows, MLP_OUT), np.float32)

@@ -212,10 +212,13 @@ struct Engine {
   double* jpart = nullptr;
   float* dts = nullptr;
   uint32_t* inv = nullptr;
-  float4* dv0 = nullptr;   // zero drift correction (network steps have no XSPH pass)
+  float4* accx = nullptr;  // network position-update acceleration (acc holds the velocity one)
+  bool sur_pred_valid = false;
+  float sur_tau = 0.0f;  // cycle phase of the current state
   float4* xref = nullptr;  // positions by original id (verification)
   MlpBuf model;
   bool has_model = false;
+  bool held_mode = false;  // no network: a = a_last
   Snapshot sur_ck;
 
   // host mirror (original order)
@@ -253,7 +256,7 @@ struct Engine {
     tune_ck.release();
     sur_ck.release();
     dfree(aux[0]); dfree(aux[1]); dfree(sr); dfree(feat); dfree(x0); dfree(jpart); dfree(dts); dfree(inv);
-    dfree(dv0); dfree(xref);
+    dfree(accx); dfree(xref);
     model.release();
     if (h_stage) { (void)hipHostFree(h_stage); h_stage = nullptr; }
     if (st) { (void)hipStreamDestroy(st); st = nullptr; }
@@ -703,12 +706,14 @@ struct Engine {
   // lib.rs:300-334 with the dt on the device; E (UBP_NKERN + 1 events)
   // brackets the kernels in the order documented for ubp_step_n_profiled
   void step_body(int do_cfl, hipEvent_t* E = nullptr) {
+    sur_pred_valid = false;
     rec(E, 0);
     UBP_LAUNCH_M(PASS_ELEM, k_kick, n, vel, acc, pos, rho_n, dtp, xr);
     rec(E, 1);
     UBP_LAUNCH_M(PASS_XSPH, k_xsph, n, xr, start, nbr, nn, dv);
     rec(E, 2);
-    UBP_LAUNCH_M(PASS_ELEM, k_drift, n, pos, vel, dv, dtp, pos_t, vel_t, wall_t, key, arr, cnt);
+    UBP_LAUNCH_M(PASS_ELEM, k_drift, n, pos, vel, dv, dtp, pos_t, vel_t, wall_t, key, arr, cnt,
+                 (const float4*)nullptr, (const float4*)nullptr);
     rec(E, 3);
     bin(E);
     density_forces(E);
@@ -1068,16 +1073,21 @@ struct Engine {
     dalloc(&jpart, 3 * ((n + 255) / 256 + 1));
     dalloc(&dts, 4);
     dalloc(&inv, n);
-    dalloc(&dv0, n);
-    HIPCHK(hipMemset(dv0, 0, n * 16));
+    dalloc(&accx, n);
     dalloc(&xref, n);
     sur_on = true;
     invalidate_graph();  // the gather now carries the surrogate state
   }
 
+  // m == nullptr selects the network-free baseline a = a_last
   void sur_set_model(const ubp_mlp_model* m) {
-    model.upload(m, st);
-    has_model = true;
+    if (m) {
+      model.upload(m, st);
+      has_model = true;
+      held_mode = false;
+    } else {
+      held_mode = true;
+    }
   }
 
   // cycle start: a_last := a, delta_last := rho / rho0 - 1, x0 := x
@@ -1086,6 +1096,14 @@ struct Engine {
     if (needs_init) throw Error(UBP_EINVAL, "the surrogate needs a stepped state (no step yet)");
     launch(PASS_ELEM, k_aux_set, n, P, (const float4*)acc, (const float*)rho_n, (const float4*)pos,
            (const uint8_t*)ft[pp], (const uint32_t*)orig[pp], aux[pp], sr, x0);
+    sur_pred_valid = false;
+    sur_tau = 0.0f;
+  }
+
+  // features of the current state with the full velocities
+  void sur_features_full(float tau) {
+    launch(PASS_ELEM, k_sr_vel, n, uint32_t(n), (const float4*)vel, sr);
+    sur_features(tau, nullptr);
   }
 
   void sur_features(float tau, float* rho_out) {
@@ -1094,24 +1112,34 @@ struct Engine {
            rho_out);
   }
 
-  void sur_predict(float4* out) {
+  // network outputs: out_v = velocity-update acceleration, out_x = position-update one
+  void sur_predict(float4* out_v, float4* out_x) {
+    if (held_mode) {
+      launch(PASS_ELEM, k_held, n, uint32_t(n), (const float4*)aux[pp], out_v, out_x);
+      return;
+    }
     if (!has_model) throw Error(UBP_EINVAL, "no surrogate model set");
     hipLaunchKernelGGL(k_mlp_fwd<1>, dim3(mlp_fwd_grid(n)), dim3(256), 0, st, model.view(), (const float*)feat,
-                       uint32_t(n), (float*)nullptr, (const float4*)aux[pp], out, gmag());
+                       uint32_t(n), (float*)nullptr, (const float4*)aux[pp], out_v, out_x, gmag());
     LAUNCHCHK("k_mlp_fwd");
   }
 
-  // One KDK step with the network's acceleration; dt in dts.  The network
-  // predicts the effective acceleration over the step (it absorbs the XSPH
-  // drift correction, so there is no XSPH pass); the same binning,
-  // neighbour lists and walls as the plain step, with the density and force
-  // passes replaced by the feature pass and the network.
+  // One network step of dt (in dts) from the predictions at the current
+  // state: x += dt v + dt^2 a_x / 2, v += dt a_v (the network's effective
+  // accelerations over the step absorb the XSPH drift correction, so there is
+  // no XSPH pass); walls, binning and neighbour lists as in the plain step;
+  // then the feature pass and the network give the predictions at the new
+  // state.
   void sur_step(float tau, hipEvent_t* E = nullptr) {
+    if (!sur_pred_valid) {  // predictions at the current state first
+      sur_features_full(sur_tau);
+      sur_predict(acc, accx);
+    }
     rec(E, 0);
-    UBP_LAUNCH_M(PASS_ELEM, k_kick, n, vel, acc, pos, rho_n, dts, xr);
     rec(E, 1);
     rec(E, 2);
-    UBP_LAUNCH_M(PASS_ELEM, k_drift, n, pos, vel, dv0, dts, pos_t, vel_t, wall_t, key, arr, cnt);
+    UBP_LAUNCH_M(PASS_ELEM, k_drift, n, pos, vel, dv, dts, pos_t, vel_t, wall_t, key, arr, cnt,
+                 (const float4*)accx, (const float4*)acc);
     rec(E, 3);
     bin(E);
     rec(E, 6);
@@ -1119,17 +1147,18 @@ struct Engine {
     rec(E, 8);
     sur_features(tau, rho_n);
     rec(E, 9);
-    sur_predict(acc);
+    sur_predict(acc, accx);
+    sur_pred_valid = true;
+    sur_tau = tau;
     rec(E, 10);
-    UBP_LAUNCH_M(PASS_ELEM, k_kick2_cfl, n, vel, acc, dts, red, sacc, 1, 0);
     rec(E, 11);
   }
 
   struct Judge { double err2, held2, disp2; };
   // network prediction at the current (plain) state vs the plain acceleration
   Judge sur_judge(float tau) {
-    sur_features(tau, nullptr);
-    sur_predict(dv);
+    sur_features_full(tau);
+    sur_predict(dv, xr);
     const uint32_t g = uint32_t((n + 255) / 256);
     hipLaunchKernelGGL(k_judge, dim3(g), dim3(256), 0, st, P, (const float4*)acc, (const float4*)dv,
                        (const float4*)aux[pp], (const float4*)pos, (const uint32_t*)orig[pp], (const float4*)x0,
@@ -1160,7 +1189,7 @@ struct Engine {
   // switches to plain cycles until the next verification is due.
   // dt > 0: fixed plain dt; dt <= 0: the device CFL dt.
   void sur_run(int ncycles, int K, float dt, float factor, float thr, int verify_every, ubp_surrogate_stats* S) {
-    if (!has_model) throw Error(UBP_EINVAL, "no surrogate model set");
+    if (!has_model && !held_mode) throw Error(UBP_EINVAL, "no surrogate model set");
     if (K < 1 || factor <= 0.0f || ncycles < 0 || verify_every < 0)
       throw Error(UBP_EINVAL, "bad surrogate cycle parameters");
     sur_enable();
@@ -1183,6 +1212,12 @@ struct Engine {
       return t;
     };
     if (needs_init) S->sim_time += plain(1);
+    // guard: a corrected state whose density deviation exceeds 4x the plain
+    // solver's level at the start (at least 5%) is rewound and rerun plain
+    ubp_error_metrics_t m0{};
+    metrics(&m0);
+    const double guard = std::max(0.05, 4.0 * double(m0.max_density_variation));
+    S->guard = guard;
     double est_sum = 0.0, held_sum = 0.0, err_sum = 0.0;
     int cool = 0;
     for (int c = 0; c < ncycles; ++c) {
@@ -1194,7 +1229,7 @@ struct Engine {
       }
       const bool verify = verify_every > 0 && (c % verify_every) == 0;
       sur_mark();
-      if (verify) save_snap(sur_ck);
+      save_snap(sur_ck);
       float dtc = dt;
       if (dt > 0.0f) {
         hipLaunchKernelGGL(k_set_dt, dim3(1), dim3(64), 0, st, dts, factor * dt, 1u);
@@ -1204,14 +1239,24 @@ struct Engine {
         hipLaunchKernelGGL(k_scale_dt, dim3(1), dim3(64), 0, st, dts, (const float*)dtp, factor);
       }
       LAUNCHCHK("surrogate dt");
-      // the first network step starts from the network's acceleration at
-      // the marked state (a_last itself is instantaneous, not effective)
-      sur_features(0.0f, nullptr);
-      sur_predict(acc);
+      // predictions at the marked state, then the network steps
+      sur_features_full(0.0f);
+      sur_predict(acc, accx);
+      sur_pred_valid = true;
       for (int s = 1; s <= K; ++s) sur_step(float(s) / float(K));
+      sur_pred_valid = false;  // the plain step replaces acc
       S->surrogate_steps += uint32_t(K);
       const double tnet = double(K) * double(factor) * double(dtc);
       const double tc = tnet + plain(1);
+      ubp_error_metrics_t mc{};
+      metrics(&mc);
+      if (!(double(mc.max_density_variation) <= guard)) {  // NaN trips too
+        S->guard_trips += 1;
+        restore_snap(sur_ck);
+        S->sim_time += plain(nplain_cycle);
+        cool = verify_every > 1 ? verify_every - 1 : 0;
+        continue;
+      }
       const Judge J = sur_judge(1.0f);
       const double nd = double(n);
       const double disp = std::sqrt(J.disp2 / nd);
@@ -1262,7 +1307,7 @@ struct Engine {
     if (count == 0) return first;
     for (size_t k = 0; k < count; ++k)
       if (ids[k] >= n) throw Error(UBP_EINVAL, "particle id out of range");
-    sur_features(tau, nullptr);
+    sur_features_full(tau);
     launch(PASS_ELEM, k_inverse, n, (const uint32_t*)orig[pp], uint32_t(n), inv);
     DevArr<uint32_t> d_ids(count);
     HIPCHK(hipMemcpyAsync(d_ids.p, ids, count * 4, hipMemcpyHostToDevice, st));
@@ -1285,7 +1330,8 @@ struct Engine {
     DevArr<uint32_t> d_ids(count);
     HIPCHK(hipMemcpyAsync(d_ids.p, ids, count * 4, hipMemcpyHostToDevice, st));
     launch(PASS_ELEM, k_target, count, (const uint32_t*)d_ids.p, uint32_t(count), uint32_t(n), (const uint32_t*)inv,
-           (const float4*)pos, (const float4*)(T.pend + first * 3), win, 1.0f / gmag(), T.T + first * 4);
+           (const float4*)pos, (const float4*)vel, (const float4*)(T.pend + first * 3), win, 1.0f / gmag(),
+           T.T + first * TSTRIDE);
     sync();
     T.unfinished -= count;
   }
@@ -1293,7 +1339,7 @@ struct Engine {
   // features of the current state in original order (out: n x 32)
   void features_host(float tau, float* out) {
     if (!sur_on || needs_init) throw Error(UBP_EINVAL, "features need a marked, stepped engine");
-    sur_features(tau, nullptr);
+    sur_features_full(tau);
     std::vector<float> hf(32 * n);
     std::vector<uint32_t> ho(n);
     HIPCHK(hipMemcpyAsync(hf.data(), feat, hf.size() * 4, hipMemcpyDeviceToHost, st));
@@ -1670,7 +1716,6 @@ int ubp_set_tuning_ex(ubp_engine* e, const ubp_tuning* t) {
 int ubp_surrogate_set_model(ubp_engine* e, const ubp_mlp_model* m) {
   UBP_API_BEGIN
   UBP_HANDLE_CHECK(e);
-  if (!m) throw Error(UBP_EINVAL, "null model");
   if (e->eng.multi) throw Error(UBP_EINVAL, "the surrogate runs on one-scene engines");
   e->eng.sur_set_model(m);
   return UBP_OK;
@@ -1810,7 +1855,8 @@ int ubp_mlp_forward(const ubp_mlp_model* m, const float* x, size_t rows, float* 
   ubp::DevArr<float> dx(rows * ubp::MLP_IN), dy(rows * ubp::MLP_OUT);
   HIPCHK(hipMemcpyAsync(dx.p, x, rows * ubp::MLP_IN * 4, hipMemcpyHostToDevice, S.s));
   hipLaunchKernelGGL(ubp::k_mlp_fwd<0>, dim3(ubp::mlp_fwd_grid(rows)), dim3(256), 0, S.s, M.view(),
-                     (const float*)dx.p, uint32_t(rows), dy.p, (const float4*)nullptr, (float4*)nullptr, 0.0f);
+                     (const float*)dx.p, uint32_t(rows), dy.p, (const float4*)nullptr, (float4*)nullptr,
+                     (float4*)nullptr, 0.0f);
   LAUNCHCHK("k_mlp_fwd");
   HIPCHK(hipMemcpyAsync(y, dy.p, rows * ubp::MLP_OUT * 4, hipMemcpyDeviceToHost, S.s));
   HIPCHK(hipStreamSynchronize(S.s));
@@ -1828,16 +1874,16 @@ int ubp_mlp_loss_grad(const ubp_mlp_model* m, const float* x, const float* t, si
   ubp::MlpBuf M;
   M.upload(m, S.s);
   const uint32_t nblk = uint32_t((rows + ubp::TR - 1) / ubp::TR);
-  ubp::DevArr<float> dx(rows * ubp::MLP_IN), dt(rows * 4), part(size_t(nblk) * (ubp::MLP_NPARAM + 1)),
+  ubp::DevArr<float> dx(rows * ubp::MLP_IN), dt(rows * ubp::TSTRIDE), part(size_t(nblk) * (ubp::MLP_NPARAM + 1)),
       dg(ubp::MLP_NPARAM + 1);
   ubp::DevArr<uint32_t> didx(rows);
-  std::vector<float> ht(rows * 4, 0.0f);
+  std::vector<float> ht(rows * ubp::TSTRIDE, 0.0f);
   for (size_t r = 0; r < rows; ++r)
-    for (int c = 0; c < 3; ++c) ht[r * 4 + c] = t[r * 3 + c];
+    for (int c = 0; c < ubp::MLP_OUT; ++c) ht[r * ubp::TSTRIDE + c] = t[r * ubp::MLP_OUT + c];
   std::vector<uint32_t> hi(rows);
   for (size_t r = 0; r < rows; ++r) hi[r] = uint32_t(r);
   HIPCHK(hipMemcpyAsync(dx.p, x, rows * ubp::MLP_IN * 4, hipMemcpyHostToDevice, S.s));
-  HIPCHK(hipMemcpyAsync(dt.p, ht.data(), rows * 16, hipMemcpyHostToDevice, S.s));
+  HIPCHK(hipMemcpyAsync(dt.p, ht.data(), ht.size() * 4, hipMemcpyHostToDevice, S.s));
   HIPCHK(hipMemcpyAsync(didx.p, hi.data(), rows * 4, hipMemcpyHostToDevice, S.s));
   ubp::mlp_grad(M, dx.p, dt.p, didx.p, uint32_t(rows), part.p, dg.p, S.s);
   std::vector<float> hg(ubp::MLP_NPARAM + 1);
@@ -1845,7 +1891,7 @@ int ubp_mlp_loss_grad(const ubp_mlp_model* m, const float* x, const float* t, si
   HIPCHK(hipStreamSynchronize(S.s));
   std::memset(grad, 0, sizeof(*grad));
   std::memcpy(grad, hg.data(), ubp::MLP_NPARAM * 4);
-  if (loss) *loss = double(hg[ubp::MLP_NPARAM]) / (3.0 * double(rows));
+  if (loss) *loss = double(hg[ubp::MLP_NPARAM]) / (double(ubp::MLP_OUT) * double(rows));
   return UBP_OK;
   UBP_API_END
 }

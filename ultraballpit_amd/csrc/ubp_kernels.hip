@@ -885,7 +885,8 @@ __global__ UBP_LB void k_cfl_finalize(DevParams P, const uint32_t* __restrict__ 
 //   s11..13 = sum V_j W_ij (a_j - a_i)   (a, delta: state at the last correction)
 // scaled to O(1) by h, 1/U (U = c_s / 10) and 1/|g|; also v_i, a_last_i,
 // delta_last_i, the density change since the last correction, the SDF wall
-// distance / normal and the phase tau within the correction cycle.
+// distance / normal, the phase tau within the correction cycle and the
+// resolution (h in units of 1 cm).
 // rho_out (optional) receives the summation density rho0 s0.
 __global__ UBP_LB void k_feat(DevParams P, const float4* __restrict__ sr, const float4* __restrict__ wall,
                               const uint8_t* __restrict__ ft, const uint32_t* __restrict__ start,
@@ -956,7 +957,7 @@ __global__ UBP_LB void k_feat(DevParams P, const float4* __restrict__ sr, const 
   o[3] = make_float4(s12 * inv_g, s13 * inv_g, vi.x * inv_u, vi.y * inv_u);
   o[4] = make_float4(vi.z * inv_u, ai.x * inv_g, ai.y * inv_g, ai.z * inv_g);
   o[5] = make_float4(vi.w * 100.0f, (d0 - vi.w) * 100.0f, sw, wx);
-  o[6] = make_float4(wy, wz, tau, 0.0f);
+  o[6] = make_float4(wy, wz, tau, h * 100.0f);
   o[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
@@ -1039,24 +1040,53 @@ __global__ UBP_LB void k_collect(const uint32_t* __restrict__ ids, uint32_t coun
   pend[3u * k + 2u] = make_float4(a.x, a.y, a.z, 0.0f);
 }
 
-// Second half, a window `win` later: the effective acceleration over the
-// window, a_eff = 2 (x(t + win) - x(t) - win v(t)) / win^2 (the constant
-// acceleration that moves the particle from x(t) with v(t) to x(t + win)),
-// and the target (a_eff - a_last) / |g|.
+// Second half, a window `win` later: the effective accelerations over the
+// window, for positions a_x = 2 (x(t + win) - x(t) - win v(t)) / win^2 (the
+// constant acceleration that carries the particle from x(t) with v(t) to
+// x(t + win); it includes the XSPH drift correction) and for velocities
+// a_v = (v(t + win) - v(t)) / win; targets (a_x, a_v) / |g|.
 __global__ UBP_LB void k_target(const uint32_t* __restrict__ ids, uint32_t count, uint32_t n,
                                 const uint32_t* __restrict__ inv, const float4* __restrict__ pos,
-                                const float4* __restrict__ pend, float win, float inv_g, float* __restrict__ T) {
+                                const float4* __restrict__ vel, const float4* __restrict__ pend, float win,
+                                float inv_g, float* __restrict__ T) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= count) return;
   const uint32_t id = ids[k];
   if (id >= n) return;
-  const float4 x1 = pos[inv[id]];
-  const float4 x0 = pend[3u * k], v0 = pend[3u * k + 1u], al = pend[3u * k + 2u];
+  const uint32_t s = inv[id];
+  const float4 x1 = pos[s], v1 = vel[s];
+  const float4 x0 = pend[3u * k], v0 = pend[3u * k + 1u];
   const float c = 2.0f / (win * win);
   const float ax = c * ((x1.x - x0.x) - win * v0.x);
   const float ay = c * ((x1.y - x0.y) - win * v0.y);
   const float az = c * ((x1.z - x0.z) - win * v0.z);
-  reinterpret_cast<float4*>(T)[k] = make_float4((ax - al.x) * inv_g, (ay - al.y) * inv_g, (az - al.z) * inv_g, 0.0f);
+  const float bx = (v1.x - v0.x) / win, by = (v1.y - v0.y) / win, bz = (v1.z - v0.z) / win;
+  float4* t = reinterpret_cast<float4*>(T) + 2u * k;
+  t[0] = make_float4(ax * inv_g, ay * inv_g, az * inv_g, bx * inv_g);
+  t[1] = make_float4(by * inv_g, bz * inv_g, 0.0f, 0.0f);
+}
+
+// network-free baseline: both step accelerations = a_last
+__global__ UBP_LB void k_held(uint32_t n, const float4* __restrict__ aux, float4* __restrict__ acc,
+                              float4* __restrict__ accx) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 a = aux[i];
+  const float4 o = make_float4(a.x, a.y, a.z, 0.0f);
+  acc[i] = o;
+  accx[i] = o;
+}
+
+// feature-record velocity := the full velocity (after the second kick)
+__global__ UBP_LB void k_sr_vel(uint32_t n, const float4* __restrict__ vel, float4* __restrict__ sr) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 v = vel[i];
+  float4 r = sr[3u * i + 1u];
+  r.x = v.x;
+  r.y = v.y;
+  r.z = v.z;
+  sr[3u * i + 1u] = r;
 }
 
 // Sums of |pos - ref[orig]| and |pos - x0[orig]| (per-block FP64, 256 threads):
@@ -1118,14 +1148,16 @@ __global__ __launch_bounds__(256) void k_colstats(const float* __restrict__ A, u
   }
 }
 
-// Tn = (T - mean) / std for the three target columns
+// Tn = (T - mean) / std for the six target columns (rows of 8 floats)
 __global__ UBP_LB void k_norm_targets(const float* __restrict__ T, uint32_t rows, const float* __restrict__ mean,
                                       const float* __restrict__ sd, float* __restrict__ Tn) {
   const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= rows) return;
-  const float4 t = reinterpret_cast<const float4*>(T)[r];
-  reinterpret_cast<float4*>(Tn)[r] =
-      make_float4((t.x - mean[0]) / sd[0], (t.y - mean[1]) / sd[1], (t.z - mean[2]) / sd[2], 0.0f);
+  const float4 a = reinterpret_cast<const float4*>(T)[2u * r];
+  const float4 b = reinterpret_cast<const float4*>(T)[2u * r + 1u];
+  reinterpret_cast<float4*>(Tn)[2u * r] = make_float4((a.x - mean[0]) / sd[0], (a.y - mean[1]) / sd[1],
+                                                      (a.z - mean[2]) / sd[2], (a.w - mean[3]) / sd[3]);
+  reinterpret_cast<float4*>(Tn)[2u * r + 1u] = make_float4((b.x - mean[4]) / sd[4], (b.y - mean[5]) / sd[5], 0.0f, 0.0f);
 }
 
 // Per-scene dt (many-scene mode): the 64-thread workgroup s reduces the
@@ -1218,18 +1250,34 @@ __global__ UBP_LB void k_drift(DevParams P0, Scenes S, const float4* __restrict_
                                const float* __restrict__ dtp, float4* __restrict__ pos_t,
                                float4* __restrict__ vel_t, float4* __restrict__ wall_t,
                                uint32_t* __restrict__ key, uint32_t* __restrict__ arr,
-                               uint32_t* __restrict__ cnt) {
+                               uint32_t* __restrict__ cnt, const float4* __restrict__ sax,
+                               const float4* __restrict__ sav) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P0.n) return;
   UBP_SCENE(i);
   const float dt = dtp[sc_];
   const float4 pi = pos[i];
   const float4 vi = vel[i];
-  const float4 d = dv[i];
-  float x = pi.x + (vi.x + d.x) * dt;
-  float y = pi.y + (vi.y + d.y) * dt;
-  float z = pi.z + (vi.z + d.z) * dt;
-  float vx = vi.x, vy = vi.y, vz = vi.z;
+  float x, y, z, vx, vy, vz;
+  if (sax) {
+    // network step: x += dt v + dt^2 a_x / 2, v += dt a_v
+    const float4 ax = sax[i], av = sav[i];
+    const float hdt = 0.5f * dt;
+    x = pi.x + (vi.x + ax.x * hdt) * dt;
+    y = pi.y + (vi.y + ax.y * hdt) * dt;
+    z = pi.z + (vi.z + ax.z * hdt) * dt;
+    vx = vi.x + av.x * dt;
+    vy = vi.y + av.y * dt;
+    vz = vi.z + av.z * dt;
+  } else {
+    const float4 d = dv[i];
+    x = pi.x + (vi.x + d.x) * dt;
+    y = pi.y + (vi.y + d.y) * dt;
+    z = pi.z + (vi.z + d.z) * dt;
+    vx = vi.x;
+    vy = vi.y;
+    vz = vi.z;
+  }
   // Wall record at the drifted position; it is also the record of the final
   // position unless the SDF projection or the domain clamp moved the particle,
   // in which case it is evaluated once more (one call site, inlined).

@@ -98,7 +98,7 @@ struct Trainer {
   int dev = 0;
   size_t cap = 0, rows = 0;
   float* X = nullptr;   // [cap][32] raw features
-  float* T = nullptr;   // [cap][4] raw targets (3 used), normalised at fit time
+  float* T = nullptr;   // [cap][8] raw targets (6 used), normalised at fit time
   float* Tn = nullptr;  // normalised targets
   float4* pend = nullptr;  // [cap][3] (x, v, a_last) of rows awaiting their target
   size_t unfinished = 0;
@@ -175,7 +175,7 @@ struct Trainer {
         m->in_mean[c] = float(mu[c]);
         m->in_inv_std[c] = sd[c] > 1e-12 ? float(1.0 / sd[c]) : 0.0f;
       }
-      colstats(T, MLP_OUT, 4, mu, sd);
+      colstats(T, MLP_OUT, TSTRIDE, mu, sd);
       for (int c = 0; c < MLP_OUT; ++c) {
         m->out_mean[c] = float(mu[c]);
         m->out_std[c] = sd[c] > 1e-12 ? float(sd[c]) : 1.0f;
@@ -229,7 +229,7 @@ struct Trainer {
     HIPCHK(hipMemcpyAsync(h.data(), dl.p, h.size() * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     if (losses)
-      for (int s = 0; s < steps; ++s) losses[s] = float(double(h[s]) / (3.0 * double(batch)));
+      for (int s = 0; s < steps; ++s) losses[s] = float(double(h[s]) / (double(MLP_OUT) * double(batch)));
   }
 
   void create(size_t capacity, uint64_t sd) {
@@ -242,8 +242,8 @@ struct Trainer {
     cap = capacity;
     seed = sd;
     dalloc(&X, cap * MLP_IN);
-    dalloc(&T, cap * 4);
-    dalloc(&Tn, cap * 4);
+    dalloc(&T, cap * TSTRIDE);
+    dalloc(&Tn, cap * TSTRIDE);
     dalloc(&pend, cap * 3);
     M.alloc();
     dalloc(&m1, MLP_NPARAM);

@@ -1,6 +1,6 @@
 // ubp_mlp.hip -- BF16-MFMA kernels of the surrogate network (gfx950).
 //
-// Network: 32 -> 128 -> 128 -> 128 -> 3, ReLU, BF16 operands, FP32
+// Network: 32 -> 128 -> 128 -> 128 -> 6, ReLU, BF16 operands, FP32
 // accumulation (v_mfma_f32_16x16x32_bf16).  Weights live in device memory as
 // pre-packed MFMA B-operand fragments, so every lane loads its operand as one
 // 16-byte vector:
@@ -19,7 +19,8 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int MLP_IN = 32, MLP_HID = 128, MLP_OUT = 3;
+constexpr int MLP_IN = 32, MLP_HID = 128, MLP_OUT = 6;
+constexpr int TSTRIDE = 8;  // floats per target row (MLP_OUT used)
 // parameter vector offsets (the field order of ubp_mlp_model)
 constexpr int OFF_W1 = 0;
 constexpr int OFF_B1 = OFF_W1 + MLP_IN * MLP_HID;
@@ -34,10 +35,10 @@ constexpr int MLP_NPARAM = OFF_B4 + MLP_OUT;
 constexpr int FR_W1 = 0;                       // W1: K 32, N 128: 8 x 1
 constexpr int FR_W2 = FR_W1 + 8 * 1 * 512;     // W2: 8 x 4
 constexpr int FR_W3 = FR_W2 + 8 * 4 * 512;     // W3: 8 x 4
-constexpr int FR_W4 = FR_W3 + 8 * 4 * 512;     // W4: K 128, N 16 (3 used): 1 x 4
+constexpr int FR_W4 = FR_W3 + 8 * 4 * 512;     // W4: K 128, N 16 (6 used): 1 x 4
 constexpr int FR_T2 = FR_W4 + 1 * 4 * 512;     // W2^T: 8 x 4
 constexpr int FR_T3 = FR_T2 + 8 * 4 * 512;     // W3^T: 8 x 4
-constexpr int FR_T4 = FR_T3 + 8 * 4 * 512;     // W4^T: K 32 (3 used), N 128: 8 x 1
+constexpr int FR_T4 = FR_T3 + 8 * 4 * 512;     // W4^T: K 32 (6 used), N 128: 8 x 1
 constexpr int FR_TOTAL = FR_T4 + 8 * 1 * 512;
 
 // Device view of a model: packed BF16 fragments + FP32 biases and
@@ -47,8 +48,8 @@ struct MlpDev {
   const float* prm;       // FP32 parameters (biases read from here)
   const float* in_mean;   // [32]
   const float* in_istd;   // [32]
-  const float* out_mean;  // [3]
-  const float* out_std;   // [3]
+  const float* out_mean;  // [6]
+  const float* out_std;   // [6]
 };
 
 __device__ __forceinline__ uint16_t f2bf(float f) {
@@ -118,12 +119,13 @@ constexpr int HS = 136;  // 128 hidden
 // Inference.  Persistent workgroups of 4 waves walk 64-row tiles.  Wave w
 // computes hidden columns [32 w, 32 w + 32) of every layer with its weight
 // fragments held in registers, and row tile w of the output layer.
-//   MODE 0: Y[row][3] = y (normalised network output);
-//   MODE 1: acc[row] = a_last + |g| (out_mean + out_std y), a_last = aux.xyz.
+//   MODE 0: Y[row][6] = y (normalised network output);
+//   MODE 1: with a = |g| (out_mean + out_std y): accx[row] = a[0..2]
+//           (position update), acc[row] = a[3..5] (velocity update).
 template <int MODE>
 __global__ __launch_bounds__(256) void k_mlp_fwd(MlpDev W, const float* __restrict__ X, uint32_t rows,
                                                  float* __restrict__ Y, const float4* __restrict__ aux,
-                                                 float4* __restrict__ acc, float gmag) {
+                                                 float4* __restrict__ acc, float4* __restrict__ accx, float gmag) {
   __shared__ __attribute__((aligned(16))) uint16_t Xs[64 * XS];
   __shared__ __attribute__((aligned(16))) uint16_t Ha[64 * HS];
   __shared__ __attribute__((aligned(16))) uint16_t Hb[64 * HS];
@@ -151,9 +153,9 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(MlpDev W, const float* __restri
   float mu[8], is[8];
 #pragma unroll
   for (int e = 0; e < 8; ++e) { mu[e] = W.in_mean[fc + e]; is[e] = W.in_istd[fc + e]; }
-  float om[3], os[3];
+  float om[MLP_OUT], os[MLP_OUT];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) { om[c] = W.out_mean[c]; os[c] = W.out_std[c]; }
+  for (int c = 0; c < MLP_OUT; ++c) { om[c] = W.out_mean[c]; os[c] = W.out_std[c]; }
 
   for (uint32_t base = blockIdx.x * 64u; base < rows; base += gridDim.x * 64u) {
     {  // stage the normalised BF16 input tile
@@ -230,17 +232,15 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(MlpDev W, const float* __restri
         if (MODE == 0) {
           if (l16 < MLP_OUT && row < rows) Y[size_t(row) * MLP_OUT + l16] = y;
         } else {
-          // gather the three outputs of this row into lane 16 lg
-          const float yy = __shfl(y, 16 * lg + 1, 64);
-          const float yz = __shfl(y, 16 * lg + 2, 64);
+          // gather the six outputs of this row into lane 16 lg
+          float yc[MLP_OUT];
+#pragma unroll
+          for (int c = 0; c < MLP_OUT; ++c) yc[c] = __shfl(y, 16 * lg + c, 64);
           if (l16 == 0 && row < rows) {
-            const float4 al = aux[row];
-            float4 o;
-            o.x = al.x + gmag * (om[0] + os[0] * y);
-            o.y = al.y + gmag * (om[1] + os[1] * yy);
-            o.z = al.z + gmag * (om[2] + os[2] * yz);
-            o.w = 0.0f;
-            acc[row] = o;
+            accx[row] = make_float4(gmag * (om[0] + os[0] * yc[0]), gmag * (om[1] + os[1] * yc[1]),
+                                    gmag * (om[2] + os[2] * yc[2]), 0.0f);
+            acc[row] = make_float4(gmag * (om[3] + os[3] * yc[3]), gmag * (om[4] + os[4] * yc[4]),
+                                   gmag * (om[5] + os[5] * yc[5]), 0.0f);
           }
         }
       }
@@ -252,7 +252,7 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(MlpDev W, const float* __restri
 // Training: one 128-row tile per workgroup, fused forward + backward.  Rows
 // are idx[base + r] of the data set (r >= batch rows contribute nothing).
 // The workgroup writes its partial gradient (model layout) and squared-error
-// sum to part[blockIdx.x][MLP_NPARAM + 1].  dY = bf16(2 (y - t) / (3 batch)).
+// sum to part[blockIdx.x][MLP_NPARAM + 1].  dY = bf16(2 (y - t) / (6 batch)).
 constexpr int TR = 128;
 __global__ __launch_bounds__(256) void k_mlp_train(MlpDev W, const float* __restrict__ X,
                                                    const float* __restrict__ T, const uint32_t* __restrict__ idx,
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpDev W, const float* __rest
   const int l16 = lane & 15, lg = lane >> 4;
   const uint32_t base = blockIdx.x * TR;
   float* out = part + size_t(blockIdx.x) * (MLP_NPARAM + 1);
-  const float scale = 2.0f / (3.0f * float(batch));
+  const float scale = 2.0f / (float(MLP_OUT) * float(batch));
 
   // stage inputs (2 rows per thread-quad pass) and clear dY
   for (int it = tid; it < TR * 4; it += 256) {
@@ -350,7 +350,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpDev W, const float* __rest
         const int r = 16 * rt + 4 * lg + v;
         const uint32_t g = base + uint32_t(r);
         if (l16 < MLP_OUT && g < batch) {
-          const float e = (d[v] + c4) - T[size_t(idx[g]) * 4 + l16];
+          const float e = (d[v] + c4) - T[size_t(idx[g]) * TSTRIDE + l16];
           ls += e * e;
           DY[r * XS + l16] = f2bf(e * scale);
         }
@@ -363,7 +363,7 @@ __global__ __launch_bounds__(256) void k_mlp_train(MlpDev W, const float* __rest
   if (tid == 0) out[MLP_NPARAM] = (sloss[0] + sloss[1]) + (sloss[2] + sloss[3]);
 
   // ---- backward ----
-  // dW4 = H3^T dY (128 x 3): wave w takes feature tiles 2w, 2w + 1
+  // dW4 = H3^T dY (128 x 6): wave w takes feature tiles 2w, 2w + 1
 #pragma unroll
   for (int t = 0; t < 2; ++t) {
     const int ft = 2 * w + t;

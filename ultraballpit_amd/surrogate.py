@@ -76,35 +76,85 @@ def collect_rollout(trainer: Trainer, scene: dict, cycles: int, dt: float, K: in
     e.close()
 
 
+def collect_corrective(trainer: Trainer, scene: dict, model: dict, cycles: int, dt: float, K: int = K_DEFAULT,
+                       factor: int = FACTOR_DEFAULT, frac: float = 1.0, seed: int = 0, warm_cycles: int = 0):
+    """Rows on the network's own states: from every mark the network (current
+    `model`) runs K steps and its states are collected; the engine then
+    returns to the mark (device checkpoint) and advances with plain steps,
+    and each row's target is the pair of effective accelerations that would
+    carry the network's state onto the plain trajectory one window later.
+    Training on these rows teaches the network to correct its own drift."""
+    e = HipKernel.from_scene(scene)
+    e.surrogate_set_model(model)
+    n = e.particle_count()
+    rng = np.random.default_rng(seed)
+    win = float(np.float32(dt)) * factor
+    dtw = float(np.float32(win))
+    e.step(dt)
+    for _ in range(warm_cycles * (K * factor + 1)):
+        e.step_async(dt)
+    e.sync()
+    for c in range(cycles):
+        e.surrogate_mark()
+        e.save_checkpoint()
+        rows = []
+        for s in range(0, K):
+            if s > 0:
+                e.surrogate_step(dtw, s / K)
+            ids = np.arange(n, dtype=np.uint32) if frac >= 1.0 else \
+                np.sort(rng.choice(n, size=max(1, int(frac * n)), replace=False)).astype(np.uint32)
+            rows.append((trainer.collect(e, ids, s / K), ids))
+        e.restore_checkpoint()
+        for s in range(0, K):
+            for _ in range(factor):
+                e.step_async(dt)
+            first, ids = rows[s]
+            trainer.finish(e, first, ids, win)
+        e.step_async(dt)  # the correction step of the cycle
+    e.sync()
+    e.close()
+
+
 def train(scenes: list, cycles: int, steps: int, batch: int = 32768, lr: float = 2e-3, lr_end: float = 1e-4,
           K: int = K_DEFAULT, factor: int = FACTOR_DEFAULT, frac: float = 1.0, seed: int = 1,
-          warm_cycles: int = 0, log=None):
+          warm_cycles: int = 0, rounds: int = 1, log=None):
     """Train a network on plain rollouts of `scenes` (each at its own fixed
-    dt).  Returns (model, losses, info)."""
-    rows = sum(int(len(s["x"]) * min(frac, 1.0)) + 1 for s in scenes) * cycles * (K + 1)
+    dt); each further round adds rows on the network's own states
+    (collect_corrective) and retrains on all rows.  Returns (model, losses,
+    info)."""
+    per_round = sum(int(len(s["x"]) * min(frac, 1.0)) + 1 for s in scenes) * cycles * (K + 1)
+    rows = per_round * max(1, rounds)
     tr = Trainer(rows, seed)
     t0 = time.perf_counter()
-    for k, sc in enumerate(scenes):
-        collect_rollout(tr, sc, cycles, fixed_dt(sc), K, factor, frac, seed + k, warm_cycles)
+    fit_s = 0.0
+    model = None
+    losses = np.zeros(0, np.float32)
+    for rnd in range(max(1, rounds)):
+        for k, sc in enumerate(scenes):
+            if rnd == 0:
+                collect_rollout(tr, sc, cycles, fixed_dt(sc), K, factor, frac, seed + k, warm_cycles)
+            else:
+                collect_corrective(tr, sc, model, cycles, fixed_dt(sc), K, factor, frac, seed + 100 * rnd + k,
+                                   warm_cycles)
+            if log:
+                log(f"[surrogate] round {rnd} rollout {k + 1}/{len(scenes)}: {tr.rows()} rows")
+        t1 = time.perf_counter()
+        tr.init_model(None)
+        losses = tr.fit(steps, batch, lr, lr_end)
+        fit_s += time.perf_counter() - t1
+        model = tr.model()
         if log:
-            log(f"[surrogate] rollout {k + 1}/{len(scenes)}: {tr.rows()} rows")
-    t1 = time.perf_counter()
-    tr.init_model(None)
-    losses = tr.fit(steps, batch, lr, lr_end)
+            log(f"[surrogate] round {rnd}: loss {float(losses[:10].mean()):.4f} -> {float(losses[-10:].mean()):.4f}")
     t2 = time.perf_counter()
-    model = tr.model()
     tr.close()
-    return model, losses, {"rows": rows, "collect_s": t1 - t0, "fit_s": t2 - t1,
+    return model, losses, {"rows": tr_rows_or(rows), "rounds": max(1, rounds), "collect_s": t2 - t0 - fit_s,
+                           "fit_s": fit_s,
                            "loss_first": float(losses[:10].mean()) if len(losses) else None,
                            "loss_last": float(losses[-10:].mean()) if len(losses) else None}
 
 
-def zero_model(like: dict) -> dict:
-    """The network-free baseline: y = 0, i.e. a = a_last (held acceleration)."""
-    m = {k: np.zeros_like(v) for k, v in like.items()}
-    m["in_inv_std"] = np.ones_like(like["in_inv_std"])
-    m["out_std"] = np.ones_like(like["out_std"])
-    return m
+def tr_rows_or(rows):
+    return int(rows)
 
 
 def _positions(e: HipKernel) -> np.ndarray:
@@ -116,7 +166,8 @@ def evaluate(scene: dict, model: dict, cycles: int, dt: float | None = None, K: 
              factor: int = FACTOR_DEFAULT, threshold: float = 1e30, verify_every: int = 0,
              warm_steps: int = 0) -> dict:
     """Surrogate rollout vs the plain solver over the same simulated time at
-    the same fixed dt schedule, both after `warm_steps` plain steps.
+    the same fixed dt schedule, both after `warm_steps` plain steps
+    (model=None: the network-free baseline a = a_last).
     Position error = mean |x_net - x_plain| / mean |x_plain - x_start|; wall
     times are device-synchronised."""
     if dt is None:
@@ -158,4 +209,4 @@ def evaluate(scene: dict, model: dict, cycles: int, dt: float | None = None, K: 
             "rel_err": err / disp if disp > 0 else None, "stats": st}
 
 
-__all__ = ["fixed_dt", "collect_rollout", "train", "zero_model", "evaluate", "MLP_FIELDS"]
+__all__ = ["fixed_dt", "collect_rollout", "train", "evaluate", "MLP_FIELDS"]
