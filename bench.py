@@ -180,6 +180,42 @@ def surrogate_bench(cycles: int, warm_steps: int, model_path: str):
                                                         "loss_first", "loss_last", "wall_s")}}
 
 
+def sphere_bench(steps: int):
+    """Flow past a sphere (4,194,304 lattice sites minus the sphere, c0 = 15,
+    alpha = 0.01): device-dt steps with the FP64 net pressure + viscous force
+    on the sphere extracted after every step."""
+    sc = scenes.flow_past_sphere()
+    fs = scenes.sphere_sdf(sc["sphere"]["centre"], sc["sphere"]["radius"], sc["sdf"])
+    n = len(sc["x"])
+    e = ubp.HipKernel.from_scene(sc)
+    e.set_force_sdf(fs)
+    mu = float(sc["alpha"]) * float(sc["h"]) * float(sc["speed_of_sound"]) / 10.0 * 1000.0
+    e.step_n(5, use_graph=False)
+    e.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        e.step_n(1, use_graph=False)
+        f = e.surface_forces(mu=mu, mode=1)
+    e.sync()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        e.step_n(1, use_graph=False)
+    e.sync()
+    t2 = time.perf_counter()
+    t3 = time.perf_counter()
+    for _ in range(steps):
+        e.surface_forces(mu=mu, mode=1)
+    t4 = time.perf_counter()
+    e.close()
+    return {"workload": "flow past a sphere r = 0.03 m in a 0.4 x 0.2 x 0.2 m channel, u0 = 0.5 m/s + 1% noise, "
+                        "c0 = 15, alpha = 0.01, SDF 256x128x128", "particles": n, "steps": steps,
+            "value": n * steps / (t1 - t0), "unit": "particle-steps/s (with force extraction every step)",
+            "steps_only_value": n * steps / (t2 - t1), "force_ms": (t4 - t3) * 1e3 / steps,
+            "mu": mu, "last_force": {k: [float(x) for x in f[k]] for k in ("pressure_force", "viscous_force",
+                                                                          "net_moment")},
+            "samples": f["samples"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,6 +230,7 @@ def main():
     ap.add_argument("--sweep-only", action="store_true")
     ap.add_argument("--autotune-steps", type=int, default=3)
     ap.add_argument("--surrogate-cycles", type=int, default=8, help="0 skips the surrogate leg")
+    ap.add_argument("--sphere-steps", type=int, default=40, help="0 skips the flow-past-sphere leg")
     ap.add_argument("--surrogate-warm", type=int, default=660)
     ap.add_argument("--surrogate-model", default=os.path.join(ROOT, "ultraballpit_amd", "models",
                                                               "dam_break_surrogate.npz"))
@@ -276,6 +313,9 @@ def main():
     if args.sweep_steps > 0:
         log(f"[bench] batched sweep, {args.sweep_steps} steps")
         out["sweep"] = sweep_bench(args.sweep_steps, min(args.warmup, 10), args.autotune_steps)
+    if args.sphere_steps > 0:
+        log(f"[bench] flow past sphere, {args.sphere_steps} steps with force extraction")
+        out["sphere"] = sphere_bench(args.sphere_steps)
     if args.surrogate_cycles > 0:
         log(f"[bench] surrogate leg, {args.surrogate_cycles} cycles")
         out["surrogate"] = surrogate_bench(args.surrogate_cycles, args.surrogate_warm, args.surrogate_model)

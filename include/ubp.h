@@ -230,6 +230,30 @@ int ubp_get_tuning(const ubp_engine *e, ubp_tuning *out);
 /* Workgroup size per pass (block[UBP_NPASS]); nbr_capacity 0 keeps it. */
 int ubp_set_tuning_ex(ubp_engine *e, const ubp_tuning *t);
 
+/* ---- Force of the fluid on a geometry: replaces compute_surface_forces
+ * (backend/crates/orchestrator/src/force.rs:39-129; SurfaceForce
+ * force.rs:11-16).  The geometry is an SDF grid of its own (the body whose
+ * load is measured, e.g. the sphere alone; layout as ubp_set_sdf).  Nodes
+ * with -2h <= d <= 0 and a valid normal sample the fluid within 2h through
+ * the engine's particle grid (instead of the reference's loop over all
+ * particles): Shepard-averaged pressure p and the SPH velocity gradient G;
+ * node force dA (-p n + mu (G + G^T) n), moment about the origin.  All sums
+ * in FP64.  mode 0: dA = cell^2 per band node (the reference's weighting,
+ * which counts every node of the 2h-thick band); mode 1: a thin-shell surface
+ * quadrature, dA = 2 cell^2 (1 - |d| / cell) for -cell <= d <= 0, else 0.  mu: dynamic viscosity of the viscous traction (0: pressure
+ * only, the reference's force).  Nodes whose normal stencil leaves the grid
+ * are skipped (the reference's normal is not finite there).  Synchronous. */
+typedef struct {
+  double pressure_force[3];
+  double viscous_force[3];
+  double net_force[3];
+  double net_moment[3];
+  uint64_t samples; /* band nodes that found fluid */
+} ubp_surface_force;
+int ubp_set_force_sdf(ubp_engine *e, const float origin[3], float cell, const uint32_t dims[3],
+                      const float *distances);
+int ubp_surface_forces(ubp_engine *e, float mu, int mode, ubp_surface_force *out);
+
 /* ---- Learned surrogate (no reference counterpart).
  * A per-particle network predicts accelerations from features pooled over
  * the radius-2h neighbour lists (kernel-weighted sums of relative positions,

@@ -81,6 +81,9 @@ def lib():
     L.orc_timestep.restype = C.c_double
     L.orc_step_auto.argtypes = [C.c_void_p, C.c_int, dp]
     L.orc_step_features.argtypes = [C.c_void_p, C.c_double, fp, C.c_float, fp]
+    L.orc_set_force_sdf.argtypes = [C.c_void_p, fp, C.c_float, u32p, fp]
+    L.orc_set_field.argtypes = [C.c_void_p, C.c_int, dp]
+    L.orc_surface_forces.argtypes = [C.c_void_p, C.c_double, C.c_int, dp]
     L.orc_get.argtypes = [C.c_void_p, C.c_int, dp]
     L.orc_get_boundary_pressure.argtypes = [C.c_void_p, dp]
     L.orc_error_metrics.argtypes = [C.c_void_p, dp]
@@ -220,6 +223,26 @@ class Oracle:
         lib().orc_step_features(self.h, float(dt), a.ctypes.data_as(C.POINTER(C.c_float)), float(tau),
                                 out.ctypes.data_as(C.POINTER(C.c_float)))
         return out
+
+    def set(self, field: str, values):
+        """Overwrite a state field (synthetic fields for tests)."""
+        v = np.ascontiguousarray(values, np.float64)
+        assert v.shape == (self.n,)
+        lib().orc_set_field(self.h, FIELDS.index(field), _dp(v))
+
+    def set_force_sdf(self, sdf: dict):
+        o = np.ascontiguousarray(sdf["origin"], np.float32)
+        d = np.ascontiguousarray(sdf["dims"], np.uint32)
+        v = np.ascontiguousarray(sdf["data"], np.float32).ravel()
+        lib().orc_set_force_sdf(self.h, o.ctypes.data_as(C.POINTER(C.c_float)), float(sdf["cell"]),
+                                d.ctypes.data_as(C.POINTER(C.c_uint32)), v.ctypes.data_as(C.POINTER(C.c_float)))
+
+    def surface_forces(self, mu: float = 0.0, mode: int = 0) -> dict:
+        """Force of the fluid on the force geometry (FP64 sums, all particles)."""
+        out = np.zeros(11, np.float64)
+        lib().orc_surface_forces(self.h, float(mu), int(mode), _dp(out))
+        return {"pressure_force": out[0:3], "viscous_force": out[3:6], "net_force": out[0:3] + out[3:6],
+                "net_moment": out[6:9], "samples": int(out[9]), "abs_sum": float(out[10])}
 
     def get(self, field: str) -> np.ndarray:
         out = np.empty(self.n, np.float64)

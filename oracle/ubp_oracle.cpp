@@ -35,6 +35,7 @@
 // that stays unpinned.
 
 #include <algorithm>
+#include <type_traits>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -384,6 +385,72 @@ template <class R> struct Sim {
   State<R> ckpt;
   Grid<R> ckpt_grid;
   std::vector<uint32_t> last_nbr_count;
+  Sdf<R> fsdf;  // geometry whose fluid load is measured (surface_forces)
+
+  // Force of the fluid on a geometry: the sampling of compute_surface_forces
+  // (orchestrator/src/force.rs:39-129: SDF nodes with -2h <= d <= 0, the
+  // normalised gradient, Shepard pressure over all particles within 2h in
+  // index order) plus the viscous traction mu (G + G^T) n from the SPH
+  // velocity gradient G = sum (m_j / rho_j) u_j (x) grad W, with FP64 sums;
+  // out = (F_p, F_v, M, samples, sum |F| of the nodes).
+  void surface_forces(double mu, int mode, double out[11]) const {
+    for (int c = 0; c < 11; ++c) out[c] = 0.0;
+    const R thr = R(2) * h;
+    const R area = fsdf.cell * fsdf.cell;
+    const size_t n = p.size();
+    for (uint32_t i = 0; i < fsdf.dims[0]; ++i)
+      for (uint32_t j = 0; j < fsdf.dims[1]; ++j)
+        for (uint32_t k = 0; k < fsdf.dims[2]; ++k) {
+          const R px = fsdf.origin[0] + R(i) * fsdf.cell;
+          const R py = fsdf.origin[1] + R(j) * fsdf.cell;
+          const R pz = fsdf.origin[2] + R(k) * fsdf.cell;
+          R d;
+          if (!fsdf.sample(px, py, pz, d)) continue;
+          if (std::fabs(d) > thr || d > R(0)) continue;
+          // mode 1: thin-shell weight 2 (1 - |d| / cell) on -cell <= d <= 0
+          const R dA = mode == 0 ? area : R(2) * area * (R(1) - std::fabs(d) / fsdf.cell);
+          if (!(dA > R(0))) continue;
+          R nx, ny, nz;
+          if (!fsdf.gradient(px, py, pz, nx, ny, nz)) continue;
+          if (nx == R(0) && ny == R(0) && nz == R(0)) continue;
+          double ps = 0.0, ws = 0.0, g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+          for (size_t q = 0; q < n; ++q) {
+            const R dx = p.x[q] - px, dy = p.y[q] - py, dz = p.z[q] - pz;
+            const R r = std::sqrt(dx * dx + dy * dy + dz * dz);
+            if (!(r < thr)) continue;
+            const R w = wendland<R>(r, h);
+            ps += double(p.prs[q]) * double(w);
+            ws += double(w);
+            R gx, gy, gz;
+            wendland_grad(-dx, -dy, -dz, r, h, gx, gy, gz);
+            const double vol = double(p.m[q]) / double(p.rho[q]);
+            const double ua[3] = {double(p.vx[q]), double(p.vy[q]), double(p.vz[q])};
+            const double gb[3] = {double(gx), double(gy), double(gz)};
+            for (int a = 0; a < 3; ++a)
+              for (int b = 0; b < 3; ++b) g[3 * a + b] += vol * ua[a] * gb[b];
+          }
+          if (!(ws > 1e-12)) continue;
+          const double pr = ps / ws;
+          const double n3[3] = {double(nx), double(ny), double(nz)};
+          double f[3];
+          for (int a = 0; a < 3; ++a) {
+            double t = 0.0;
+            for (int b = 0; b < 3; ++b) t += (g[3 * a + b] + g[3 * b + a]) * n3[b];
+            const double fp = -pr * n3[a] * double(dA);
+            const double fv = mu * t * double(dA);
+            out[a] += fp;
+            out[3 + a] += fv;
+            f[a] = fp + fv;
+          }
+          const double rx = px, ry = py, rz = pz;
+          out[6] += ry * f[2] - rz * f[1];
+          out[7] += rz * f[0] - rx * f[2];
+          out[8] += rx * f[1] - ry * f[0];
+          out[9] += 1.0;
+          out[10] += std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+        }
+  }
+
   // surrogate feature hook: computed at the end of step()
   const float* feat_aux = nullptr;
   float feat_tau = 0.0f;
@@ -901,6 +968,42 @@ int orc_set_sdf(void* h, const float origin[3], float cell, const uint32_t dims[
   };
   if (hd->d) set(hd->d); else set(hd->f);
   return 0;
+}
+
+// Overwrite one state field (ids as orc_get) -- synthetic fields for tests.
+void orc_set_field(void* h, int field, const double* in) {
+  Handle* hd = static_cast<Handle*>(h);
+  auto set = [&](auto* s) {
+    auto& p = s->p;
+    auto* dst = &p.x;
+    switch (field) {
+      case 0: dst = &p.x; break;   case 1: dst = &p.y; break;   case 2: dst = &p.z; break;
+      case 3: dst = &p.vx; break;  case 4: dst = &p.vy; break;  case 5: dst = &p.vz; break;
+      case 6: dst = &p.ax; break;  case 7: dst = &p.ay; break;  case 8: dst = &p.az; break;
+      case 9: dst = &p.rho; break; case 10: dst = &p.prs; break; case 11: dst = &p.m; break;
+      default: return;
+    }
+    for (size_t i = 0; i < dst->size(); ++i) (*dst)[i] = static_cast<typename std::decay_t<decltype(*dst)>::value_type>(in[i]);
+  };
+  if (hd->d) set(hd->d); else set(hd->f);
+}
+
+int orc_set_force_sdf(void* h, const float origin[3], float cell, const uint32_t dims[3], const float* data) {
+  Handle* hd = static_cast<Handle*>(h);
+  size_t nn = size_t(dims[0]) * dims[1] * dims[2];
+  auto set = [&](auto* s) {
+    s->fsdf.on = true;
+    for (int a = 0; a < 3; ++a) { s->fsdf.origin[a] = origin[a]; s->fsdf.dims[a] = dims[a]; }
+    s->fsdf.cell = cell;
+    s->fsdf.data.assign(data, data + nn);
+  };
+  if (hd->d) set(hd->d); else set(hd->f);
+  return 0;
+}
+
+void orc_surface_forces(void* h, double mu, int mode, double out[11]) {
+  Handle* hd = static_cast<Handle*>(h);
+  if (hd->d) hd->d->surface_forces(mu, mode, out); else hd->f->surface_forces(mu, mode, out);
 }
 
 int orc_set_threads(int nthreads) {

@@ -120,6 +120,11 @@ class _SurStats(C.Structure):
                 ("est_mean", C.c_double), ("held_est_mean", C.c_double)]
 
 
+class _SurfaceForce(C.Structure):
+    _fields_ = [("pressure_force", C.c_double * 3), ("viscous_force", C.c_double * 3),
+                ("net_force", C.c_double * 3), ("net_moment", C.c_double * 3), ("samples", C.c_uint64)]
+
+
 class _Scene(C.Structure):
     _fields_ = [("particles", C.POINTER(_ParticlesIn)), ("h", C.c_float), ("gravity", C.c_float * 3),
                 ("speed_of_sound", C.c_float), ("cfl_number", C.c_float),
@@ -142,7 +147,7 @@ EXPORTS = ("ubp_create", "ubp_destroy", "ubp_last_error", "ubp_gpu_available",
            "ubp_mfma_selftest", "ubp_surrogate_set_model", "ubp_surrogate_run", "ubp_surrogate_mark",
            "ubp_surrogate_step", "ubp_surrogate_features", "ubp_trainer_create", "ubp_trainer_destroy",
            "ubp_trainer_collect", "ubp_trainer_finish", "ubp_trainer_rows", "ubp_trainer_init_model", "ubp_trainer_fit",
-           "ubp_trainer_get_model")
+           "ubp_trainer_get_model", "ubp_set_force_sdf", "ubp_surface_forces")
 
 #: kernel order of ubp_step_n_profiled
 KERNELS = ("k_kick", "k_xsph", "k_drift", "k_scan", "k_scatter", "k_rank_gather", "k_neigh", "k_expand",
@@ -221,6 +226,8 @@ def lib():
     L.ubp_trainer_init_model.argtypes = [C.c_void_p, C.POINTER(_MlpModel)]
     L.ubp_trainer_fit.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_float, C.c_float, fp]
     L.ubp_trainer_get_model.argtypes = [C.c_void_p, C.POINTER(_MlpModel)]
+    L.ubp_set_force_sdf.argtypes = [C.c_void_p, fp, C.c_float, u32p, fp]
+    L.ubp_surface_forces.argtypes = [C.c_void_p, C.c_float, C.c_int, C.POINTER(_SurfaceForce)]
     for name in EXPORTS:
         getattr(L, name)
     _lib = L
@@ -397,6 +404,25 @@ class HipKernel:
 
     def scene_count(self) -> int:
         return int(lib().ubp_scene_count(self._h))
+
+    # -- force on a geometry (orchestrator/src/force.rs:39-129) --------------
+    def set_force_sdf(self, sdf: dict):
+        """SDF grid of the body whose fluid load surface_forces measures."""
+        o = _f32(sdf["origin"])
+        d = np.ascontiguousarray(sdf["dims"], dtype=np.uint32)
+        v = _f32(sdf["data"]).ravel()
+        if v.size != int(np.prod(d.astype(np.int64))):
+            raise EngineError("SDF data size does not match dims")
+        _check(lib().ubp_set_force_sdf(self._h, _fp(o), float(sdf["cell"]), _u32p(d), _fp(v)), "set_force_sdf")
+
+    def surface_forces(self, mu: float = 0.0, mode: int = 0) -> dict:
+        """Net pressure / viscous force and moment (FP64) of the fluid on the
+        force geometry (SurfaceForce, force.rs:11-16)."""
+        s = _SurfaceForce()
+        _check(lib().ubp_surface_forces(self._h, float(mu), int(mode), C.byref(s)), "surface_forces")
+        return {"pressure_force": np.array(s.pressure_force), "viscous_force": np.array(s.viscous_force),
+                "net_force": np.array(s.net_force), "net_moment": np.array(s.net_moment),
+                "samples": int(s.samples)}
 
     # -- learned surrogate ----------------------------------------------------
     def surrogate_set_model(self, model: dict | None):

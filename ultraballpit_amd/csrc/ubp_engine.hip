@@ -219,6 +219,13 @@ struct Engine {
   MlpBuf model;
   bool has_model = false;
   bool held_mode = false;  // no network: a = a_last
+
+  // geometry whose fluid load is measured (force extraction)
+  float* fsdf = nullptr;
+  double* fpart = nullptr;
+  size_t fpart_n = 0;
+  DevParams G{};
+  bool has_fsdf = false;
   Snapshot sur_ck;
 
   // host mirror (original order)
@@ -257,6 +264,7 @@ struct Engine {
     sur_ck.release();
     dfree(aux[0]); dfree(aux[1]); dfree(sr); dfree(feat); dfree(x0); dfree(jpart); dfree(dts); dfree(inv);
     dfree(accx); dfree(xref);
+    dfree(fsdf); dfree(fpart);
     model.release();
     if (h_stage) { (void)hipHostFree(h_stage); h_stage = nullptr; }
     if (st) { (void)hipStreamDestroy(st); st = nullptr; }
@@ -1348,6 +1356,56 @@ struct Engine {
     for (size_t s = 0; s < n; ++s) std::memcpy(out + size_t(ho[s]) * 32, hf.data() + s * 32, 128);
   }
 
+  // ---- force on a geometry (orchestrator/src/force.rs:39-129) ----
+  void set_force_sdf(const float o[3], float cellsz, const uint32_t d[3], const float* data) {
+    if (multi) throw Error(UBP_EINVAL, "force extraction runs on one-scene engines");
+    if (!(cellsz > 0.0f) || d[0] < 2 || d[1] < 2 || d[2] < 2 || !data) throw Error(UBP_EINVAL, "invalid SDF grid");
+    const size_t nn_ = size_t(d[0]) * d[1] * d[2];
+    if (nn_ > (size_t(1) << 31)) throw Error(UBP_EOVERFLOW, "SDF grid too large");
+    HIPCHK(hipStreamSynchronize(st));
+    dfree(fsdf);
+    dalloc(&fsdf, nn_);
+    HIPCHK(hipMemcpy(fsdf, data, nn_ * 4, hipMemcpyHostToDevice));
+    G = P;
+    for (int a = 0; a < 3; ++a) { G.so[a] = o[a]; G.sd[a] = d[a]; }
+    G.scell = cellsz;
+    G.sdf = fsdf;
+    G.sdf_on = 1;
+    has_fsdf = true;
+  }
+
+  // Net pressure and viscous force (and moment about the origin) of the
+  // fluid on the geometry, FP64 sums.  mode 0: area dA = cell^2 per band
+  // node (force.rs:58); mode 1: thin-shell surface quadrature.
+  void surface_forces(float mu, int mode, ubp_surface_force* out) {
+    if (!has_fsdf) throw Error(UBP_EINVAL, "no force geometry (ubp_set_force_sdf)");
+    if (needs_init) throw Error(UBP_EINVAL, "force extraction needs a stepped state");
+    if (mode != 0 && mode != 1) throw Error(UBP_EINVAL, "mode must be 0 or 1");
+    const size_t nodes = size_t(G.sd[0]) * G.sd[1] * G.sd[2];
+    const uint32_t g = uint32_t((nodes + 255) / 256);
+    if (size_t(g) * 10 > fpart_n) {
+      dfree(fpart);
+      dalloc(&fpart, size_t(g) * 10);
+      fpart_n = size_t(g) * 10;
+    }
+    hipLaunchKernelGGL(k_surface_forces, dim3(g), dim3(256), 0, st, P, G, (const float4*)pos, (const float4*)vel,
+                       (const float*)rho_n, (const float*)prs, (const uint32_t*)start, mu, mode, fpart);
+    LAUNCHCHK("k_surface_forces");
+    std::vector<double> h(size_t(g) * 10);
+    HIPCHK(hipMemcpyAsync(h.data(), fpart, h.size() * 8, hipMemcpyDeviceToHost, st));
+    sync();
+    double s[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t b = 0; b < g; ++b)
+      for (int c = 0; c < 10; ++c) s[c] += h[size_t(b) * 10 + c];
+    for (int a = 0; a < 3; ++a) {
+      out->pressure_force[a] = s[a];
+      out->viscous_force[a] = s[3 + a];
+      out->net_force[a] = s[a] + s[3 + a];
+      out->net_moment[a] = s[6 + a];
+    }
+    out->samples = uint64_t(s[9]);
+  }
+
   void report_tuning(ubp_tuning* out, double step_ms) const {
     if (!out) return;
     for (int k = 0; k < UBP_NPASS; ++k) out->block[k] = pblock[k];
@@ -1839,6 +1897,25 @@ int ubp_trainer_get_model(ubp_trainer* t, ubp_mlp_model* out) {
   if (!out) throw Error(UBP_EINVAL, "null model");
   if (!t->tr.stats_set) throw Error(UBP_EINVAL, "no model yet (init_model)");
   t->tr.M.download(out, t->tr.st);
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_set_force_sdf(ubp_engine* e, const float origin[3], float cell, const uint32_t dims[3],
+                      const float* distances) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  if (!origin || !dims) throw Error(UBP_EINVAL, "null SDF argument");
+  e->eng.set_force_sdf(origin, cell, dims, distances);
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_surface_forces(ubp_engine* e, float mu, int mode, ubp_surface_force* out) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  if (!out) throw Error(UBP_EINVAL, "null output");
+  e->eng.surface_forces(mu, mode, out);
   return UBP_OK;
   UBP_API_END
 }

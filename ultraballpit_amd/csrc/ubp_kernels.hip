@@ -1317,6 +1317,110 @@ __global__ UBP_LB void k_drift(DevParams P0, Scenes S, const float4* __restrict_
   arr[i] = wave_histogram_add(cnt, code);
 }
 
+// ---- force on a geometry (orchestrator/src/force.rs:39-129, FP64 sums) ----
+// One thread per node (i, j, k) of the geometry's SDF grid (parameters in G:
+// so / scell / sd / sdf and h; particle grid in P).  Nodes with
+// -2h <= d <= 0 and a valid normal sample the fluid within 2h through the
+// particle cell grid: Shepard pressure p = sum P_j W / sum W and, for the
+// viscous traction, the SPH velocity gradient G = sum (m_j / rho_j) u_j (x)
+// grad W.  Node force dA (-p n + mu (G + G^T) n), moment x_node x force.
+// dA: cell^2 (mode 0, the reference's weight) or the thin-shell quadrature
+// 2 cell^2 (1 - |d| / cell) for -cell <= d <= 0 (mode 1).
+// Per-block FP64 partials part[b][10] = (F_p, F_v, M, samples); written for
+// 256 threads.
+__global__ __launch_bounds__(256) void k_surface_forces(DevParams P, DevParams G, const float4* __restrict__ pos,
+                                                        const float4* __restrict__ vel,
+                                                        const float* __restrict__ rho,
+                                                        const float* __restrict__ prs,
+                                                        const uint32_t* __restrict__ start, float mu, int mode,
+                                                        double* __restrict__ part) {
+  __shared__ double sm[10][256];
+  const uint32_t nodes = G.sd[0] * G.sd[1] * G.sd[2];
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  double out[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  do {
+    if (idx >= nodes) break;
+    const uint32_t k = idx % G.sd[2], j = (idx / G.sd[2]) % G.sd[1], i = idx / (G.sd[1] * G.sd[2]);
+    const float px = G.so[0] + float(i) * G.scell;
+    const float py = G.so[1] + float(j) * G.scell;
+    const float pz = G.so[2] + float(k) * G.scell;
+    float d;
+    if (!sdf_sample(G, px, py, pz, d)) break;
+    const float thr = 2.0f * G.h;
+    if (fabsf(d) > thr || d > 0.0f) break;
+    // mode 0: dA = cell^2 (force.rs:58); mode 1: thin-shell weight
+    // 2 cell^2 (1 - |d| / cell) on -cell <= d <= 0 (unbiased over offsets)
+    const float area = G.scell * G.scell;
+    const float dA = mode == 0 ? area : 2.0f * area * (1.0f - fabsf(d) / G.scell);
+    if (!(dA > 0.0f)) break;
+    // normalised central difference at +-cell/2 (geometry.rs:153-175)
+    const float hh = G.scell * 0.5f, two_h = 2.0f * hh;
+    float v0, v1, gxn, gyn, gzn;
+    if (!sdf_sample(G, px + hh, py, pz, v0) || !sdf_sample(G, px - hh, py, pz, v1)) break;
+    gxn = (v0 - v1) / two_h;
+    if (!sdf_sample(G, px, py + hh, pz, v0) || !sdf_sample(G, px, py - hh, pz, v1)) break;
+    gyn = (v0 - v1) / two_h;
+    if (!sdf_sample(G, px, py, pz + hh, v0) || !sdf_sample(G, px, py, pz - hh, v1)) break;
+    gzn = (v0 - v1) / two_h;
+    const float mag = sqrtf(gxn * gxn + gyn * gyn + gzn * gzn);
+    if (!(mag > 1e-8f)) break;
+    const float nx = gxn / mag, ny = gyn / mag, nz = gzn / mag;
+    uint32_t cx, cy, cz;
+    pos_to_cell(P, px, py, pz, cx, cy, cz);
+    Nbhd N;
+    nbhd(P, cx, cy, cz, N);
+    double ps = 0.0, ws = 0.0, g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int kk = 0; kk < 27; ++kk) {
+      const uint32_t sx = sel3(N.sx, kk % 3), sy = sel3(N.sy, (kk / 3) % 3), sz = sel3(N.sz, kk / 9);
+      if (sx == 0xffffffffu || sy == 0xffffffffu || sz == 0xffffffffu) continue;
+      const uint32_t code = P.code_base + (sx | sy | sz);
+      for (uint32_t s = start[code], e = start[code + 1u]; s < e; ++s) {
+        const float4 q = pos[s];
+        const float dx = q.x - px, dy = q.y - py, dz = q.z - pz;
+        const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+        if (!(r < thr)) continue;
+        const float qq = div_h(r, P);
+        const float w = wendland_q(qq, P);
+        ps += double(prs[s]) * double(w);
+        ws += double(w);
+        float gx, gy, gz;  // gradient with respect to the node position
+        wgrad_q(-dx, -dy, -dz, r, qq, P, gx, gy, gz);
+        const float4 u = vel[s];
+        const double vol = double(q.w) / double(rho[s]);
+        const double ua[3] = {u.x, u.y, u.z}, gb[3] = {gx, gy, gz};
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) g[3 * a + b] += vol * ua[a] * gb[b];
+      }
+    }
+    if (!(ws > 1e-12)) break;
+    const double p = ps / ws;
+    const double n3[3] = {nx, ny, nz};
+    double fp[3], fv[3];
+    for (int a = 0; a < 3; ++a) {
+      double t = 0.0;
+      for (int b = 0; b < 3; ++b) t += (g[3 * a + b] + g[3 * b + a]) * n3[b];
+      fp[a] = -p * n3[a] * double(dA);
+      fv[a] = double(mu) * t * double(dA);
+    }
+    const double f[3] = {fp[0] + fv[0], fp[1] + fv[1], fp[2] + fv[2]};
+    const double rx = px, ry = py, rz = pz;
+    out[0] = fp[0]; out[1] = fp[1]; out[2] = fp[2];
+    out[3] = fv[0]; out[4] = fv[1]; out[5] = fv[2];
+    out[6] = ry * f[2] - rz * f[1];
+    out[7] = rz * f[0] - rx * f[2];
+    out[8] = rx * f[1] - ry * f[0];
+    out[9] = 1.0;
+  } while (false);
+  for (int c = 0; c < 10; ++c) sm[c][threadIdx.x] = out[c];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (int(threadIdx.x) < s)
+      for (int c = 0; c < 10; ++c) sm[c][threadIdx.x] += sm[c][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 10) part[10 * blockIdx.x + threadIdx.x] = sm[threadIdx.x][0];
+}
+
 // ---- diagnostics ----
 // Plain streaming copy (HBM counter calibration).
 __global__ UBP_LB void k_calib_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n4) {

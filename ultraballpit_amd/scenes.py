@@ -93,6 +93,20 @@ def sphere_in_box_sdf(lo, hi, centre, radius, dims, cell=None):
     return s
 
 
+def sphere_sdf(centre, radius, like: dict) -> dict:
+    """Analytic sphere distance |x - c| - r on the node grid of `like`."""
+    dims = np.asarray(like["dims"]).astype(np.int64)
+    o = np.asarray(like["origin"], np.float64)
+    c = float(like["cell"])
+    gx = o[0] + c * np.arange(dims[0])
+    gy = o[1] + c * np.arange(dims[1])
+    gz = o[2] + c * np.arange(dims[2])
+    X, Y, Z = np.meshgrid(gx, gy, gz, indexing="ij")
+    d = np.sqrt((X - centre[0]) ** 2 + (Y - centre[1]) ** 2 + (Z - centre[2]) ** 2) - radius
+    return {"origin": np.asarray(like["origin"], np.float32), "cell": np.float32(like["cell"]),
+            "dims": np.asarray(like["dims"], np.uint32), "data": d.astype(np.float32).ravel()}
+
+
 def _particles(pos: np.ndarray, dx: float, rho0: float = WATER_RHO0) -> dict:
     n = len(pos)
     m = np.float32(rho0 * dx * dx * dx)
