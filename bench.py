@@ -228,6 +228,7 @@ def main():
     ap.add_argument("--sweep-steps", type=int, default=50,
                     help="steps of the 256-scene batched sweep (0 skips it)")
     ap.add_argument("--sweep-only", action="store_true")
+    ap.add_argument("--surrogate-only", action="store_true")
     ap.add_argument("--autotune-steps", type=int, default=3)
     ap.add_argument("--surrogate-cycles", type=int, default=8, help="0 skips the surrogate leg")
     ap.add_argument("--sphere-steps", type=int, default=40, help="0 skips the flow-past-sphere leg")
@@ -240,6 +241,12 @@ def main():
               "(see DESIGN.md for how the dam break would shard)", file=sys.stderr)
         sys.exit(2)
 
+    if args.surrogate_only:
+        log(f"[bench] surrogate leg, {args.surrogate_cycles} cycles")
+        out = surrogate_bench(args.surrogate_cycles, args.surrogate_warm, args.surrogate_model)
+        out["metric"] = "surrogate/plain speedup on the 1M dam break"
+        print(json.dumps(out), flush=True)
+        return
     if args.sweep_only:
         log(f"[bench] batched sweep, {args.sweep_steps} steps")
         out = sweep_bench(args.sweep_steps, args.warmup, args.autotune_steps)
