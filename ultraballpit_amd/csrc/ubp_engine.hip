@@ -1074,8 +1074,8 @@ struct Engine {
     if (sur_on) return;
     HIPCHK(hipStreamSynchronize(st));
     for (int k = 0; k < 2; ++k) { dalloc(&aux[k], n); HIPCHK(hipMemset(aux[k], 0, n * 16)); }
-    dalloc(&sr, 4 * n);  // 64-byte records (line aligned)
-    HIPCHK(hipMemset(sr, 0, 4 * n * 16));
+    dalloc(&sr, 3 * n);
+    HIPCHK(hipMemset(sr, 0, 3 * n * 16));
     dalloc(&feat, 32 * n);
     dalloc(&x0, n);
     dalloc(&jpart, 3 * ((n + 255) / 256 + 1));

@@ -195,9 +195,9 @@ __global__ UBP_LB void k_rank_gather(DevParams P0, Scenes S, const uint32_t* __r
     const float4 a = aux_src[s];
     const float4 v = vel_t[s];
     aux_dst[t] = a;
-    sr[4u * t] = p;
-    sr[4u * t + 1u] = make_float4(v.x, v.y, v.z, a.w);
-    sr[4u * t + 2u] = a;
+    sr[3u * t] = p;
+    sr[3u * t + 1u] = make_float4(v.x, v.y, v.z, a.w);
+    sr[3u * t + 2u] = a;
   }
 }
 
@@ -878,7 +878,7 @@ __global__ UBP_LB void k_cfl_finalize(DevParams P, const uint32_t* __restrict__ 
 // ---- learned surrogate ----
 // Input features of particle i, pooled over its radius-2h neighbours in the
 // reference order (plain FP32 sums), from the feature record
-// sr = (x y z m)(v, delta_last)(a_last, .)(unused) of every slot (64 bytes):
+// sr = (x y z m)(v, delta_last)(a_last, .) of every slot:
 //   s0 = sum V_j W_ij + V_i W(0) (+ wall Omega)        V_j = m_j / rho0
 //   s1..3 = sum V_j grad_i W_ij          s4..6 = sum V_j W_ij (v_j - v_i)
 //   s7 = sum V_j (v_j - v_i) . grad W    s8..10 = sum V_j (delta_j - delta_i) grad W
@@ -895,15 +895,15 @@ __global__ UBP_LB void k_feat(DevParams P, const float4* __restrict__ sr, const 
   uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   uint32_t i = blk * blockDim.x + threadIdx.x;
   if (i >= P.n) return;
-  const float4 pi = sr[4u * i];
-  const float4 vi = sr[4u * i + 1u];
-  const float4 ai = sr[4u * i + 2u];
+  const float4 pi = sr[3u * i];
+  const float4 vi = sr[3u * i + 1u];
+  const float4 ai = sr[3u * i + 2u];
   const int f = ft[i];
   const float ir0 = 1.0f / P.rho0[f];
   float s0 = pi.w * ir0 * P.norm_h3;
   float s1 = 0.0f, s2 = 0.0f, s3 = 0.0f, s4 = 0.0f, s5 = 0.0f, s6 = 0.0f, s7 = 0.0f;
   float s8 = 0.0f, s9 = 0.0f, s10 = 0.0f, s11 = 0.0f, s12 = 0.0f, s13 = 0.0f;
-  for_neighbors<4, 4>(P, i, pi, sr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
+  for_neighbors<3>(P, i, pi, sr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
     const float dx = pi.x - q.x;
     const float dy = pi.y - q.y;
     const float dz = pi.z - q.z;
@@ -973,8 +973,8 @@ __global__ UBP_LB void k_aux_set(DevParams P, const float4* __restrict__ acc, co
   const float dl = rho[i] / P.rho0[ft[i]] - 1.0f;
   const float4 v = make_float4(a.x, a.y, a.z, dl);
   aux[i] = v;
-  sr[4u * i + 1u].w = dl;
-  sr[4u * i + 2u] = v;
+  sr[3u * i + 1u].w = dl;
+  sr[3u * i + 2u] = v;
   x0[orig[i]] = pos[i];
 }
 
@@ -1082,11 +1082,11 @@ __global__ UBP_LB void k_sr_vel(uint32_t n, const float4* __restrict__ vel, floa
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4 v = vel[i];
-  float4 r = sr[4u * i + 1u];
+  float4 r = sr[3u * i + 1u];
   r.x = v.x;
   r.y = v.y;
   r.z = v.z;
-  sr[4u * i + 1u] = r;
+  sr[3u * i + 1u] = r;
 }
 
 // Sums of |pos - ref[orig]| and |pos - x0[orig]| (per-block FP64, 256 threads):
