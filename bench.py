@@ -94,9 +94,36 @@ def cpu_baseline(scene, budget_s: float):
     o.step_auto(steps)
     t2 = time.perf_counter()
     rate = n * steps / (t2 - t1)
+    # the same steps on one thread: the scalar rate of the arithmetic
+    pyoracle.set_threads(1)
+    steps1 = max(1, min(4, int(0.5 * budget_s * steps / max(threads * (t2 - t1), 1e-3))))
+    t3 = time.perf_counter()
+    o.step_auto(steps1)
+    rate1 = n * steps1 / (time.perf_counter() - t3)
+    pyoracle.set_threads(threads)
     return {"value": rate, "unit": "particle-steps/s", "cores": threads, "kind": "port",
             "sample": f"{steps} steady steps of the {n}-particle dam break after 1 bootstrap step, "
-                      f"C++ oracle (engine arithmetic), OpenMP {threads} threads"}
+                      f"C++ oracle (engine arithmetic), OpenMP {threads} threads",
+            "single_thread": {"value": rate1, "steps": steps1},
+            "host": host_cpu()}
+
+
+def host_cpu() -> dict:
+    """CPU model and the threads this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = None
+    return {"model": model, "logical_cpus": os.cpu_count(), "allowed_cpus": allowed}
 
 
 def sweep_bench(steps: int, warmup: int, trial: int):

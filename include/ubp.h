@@ -230,6 +230,18 @@ int ubp_get_tuning(const ubp_engine *e, ubp_tuning *out);
 /* Workgroup size per pass (block[UBP_NPASS]); nbr_capacity 0 keeps it. */
 int ubp_set_tuning_ex(ubp_engine *e, const ubp_tuning *t);
 
+/* ---- Viewer frame: replaces build_frame (backend/crates/server/src/ws.rs:
+ * 232-294) over a full particles() readback.  Writes the Frame message
+ * (tag 0x02, frame number u64, particle count u32, sim time f64, then per
+ * subsampled particle x y z temperature f32, fluid type u8, density ratio
+ * u16, reserved u8; little endian) into out[capacity] and its size into
+ * *size; the particles are original indices k * stride, stride =
+ * max(n / subsample_count, 1) (1 when subsample_count is 0), count =
+ * min(n / stride, subsample_count).  The frame is built on the device and only
+ * its bytes cross PCIe.  out == NULL: only *size.  Synchronous. */
+int ubp_build_frame(ubp_engine *e, uint32_t subsample_count, uint64_t frame_number, double sim_time,
+                    uint8_t *out, size_t capacity, size_t *size);
+
 /* ---- Force of the fluid on a geometry: replaces compute_surface_forces
  * (backend/crates/orchestrator/src/force.rs:39-129; SurfaceForce
  * force.rs:11-16).  The geometry is an SDF grid of its own (the body whose

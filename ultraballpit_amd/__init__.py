@@ -147,7 +147,7 @@ EXPORTS = ("ubp_create", "ubp_destroy", "ubp_last_error", "ubp_gpu_available",
            "ubp_mfma_selftest", "ubp_surrogate_set_model", "ubp_surrogate_run", "ubp_surrogate_mark",
            "ubp_surrogate_step", "ubp_surrogate_features", "ubp_trainer_create", "ubp_trainer_destroy",
            "ubp_trainer_collect", "ubp_trainer_finish", "ubp_trainer_rows", "ubp_trainer_init_model", "ubp_trainer_fit",
-           "ubp_trainer_get_model", "ubp_set_force_sdf", "ubp_surface_forces")
+           "ubp_trainer_get_model", "ubp_set_force_sdf", "ubp_surface_forces", "ubp_build_frame")
 
 #: kernel order of ubp_step_n_profiled
 KERNELS = ("k_kick", "k_xsph", "k_drift", "k_scan", "k_scatter", "k_rank_gather", "k_neigh", "k_expand",
@@ -227,6 +227,8 @@ def lib():
     L.ubp_trainer_fit.argtypes = [C.c_void_p, C.c_int, C.c_uint32, C.c_float, C.c_float, fp]
     L.ubp_trainer_get_model.argtypes = [C.c_void_p, C.POINTER(_MlpModel)]
     L.ubp_set_force_sdf.argtypes = [C.c_void_p, fp, C.c_float, u32p, fp]
+    L.ubp_build_frame.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64, C.c_double, C.POINTER(C.c_uint8),
+                                  C.c_size_t, C.POINTER(C.c_size_t)]
     L.ubp_surface_forces.argtypes = [C.c_void_p, C.c_float, C.c_int, C.POINTER(_SurfaceForce)]
     for name in EXPORTS:
         getattr(L, name)
@@ -404,6 +406,17 @@ class HipKernel:
 
     def scene_count(self) -> int:
         return int(lib().ubp_scene_count(self._h))
+
+    # -- viewer frame (backend/crates/server/src/ws.rs:232-294) -------------
+    def build_frame(self, subsample_count: int, frame_number: int, sim_time: float) -> bytes:
+        """The server's Frame message, built on the device."""
+        size = C.c_size_t(0)
+        _check(lib().ubp_build_frame(self._h, int(subsample_count), int(frame_number), float(sim_time), None, 0,
+                                     C.byref(size)), "build_frame")
+        buf = (C.c_uint8 * size.value)()
+        _check(lib().ubp_build_frame(self._h, int(subsample_count), int(frame_number), float(sim_time), buf,
+                                     size.value, C.byref(size)), "build_frame")
+        return bytes(buf)
 
     # -- force on a geometry (orchestrator/src/force.rs:39-129) --------------
     def set_force_sdf(self, sdf: dict):

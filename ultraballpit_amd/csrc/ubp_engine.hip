@@ -220,6 +220,12 @@ struct Engine {
   bool has_model = false;
   bool held_mode = false;  // no network: a = a_last
 
+  // viewer frames: temperatures on the device (original order), payload
+  float* d_temp = nullptr;
+  uint32_t* fbuf = nullptr;
+  uint32_t* finv = nullptr;
+  size_t fbuf_n = 0;
+
   // geometry whose fluid load is measured (force extraction)
   float* fsdf = nullptr;
   double* fpart = nullptr;
@@ -265,6 +271,7 @@ struct Engine {
     dfree(aux[0]); dfree(aux[1]); dfree(sr); dfree(feat); dfree(x0); dfree(jpart); dfree(dts); dfree(inv);
     dfree(accx); dfree(xref);
     dfree(fsdf); dfree(fpart);
+    dfree(d_temp); dfree(fbuf); dfree(finv);
     model.release();
     if (h_stage) { (void)hipHostFree(h_stage); h_stage = nullptr; }
     if (st) { (void)hipStreamDestroy(st); st = nullptr; }
@@ -1356,6 +1363,58 @@ struct Engine {
     for (size_t s = 0; s < n; ++s) std::memcpy(out + size_t(ho[s]) * 32, hf.data() + s * 32, 128);
   }
 
+  // ---- viewer frame (backend/crates/server/src/ws.rs:232-294) ----
+  // Frame message: tag 0x02, frame number u64, particle count u32, sim time
+  // f64, then 20 bytes per subsampled particle (original index k * stride,
+  // stride = max(n / subsample, 1), or 1 when subsample is 0, and count =
+  // min(n / stride, subsample), so subsample 0 gives an empty frame).
+  // Built on the device; only the frame crosses PCIe.  Returns the size.
+  size_t build_frame(uint32_t subsample, uint64_t frame_number, double sim_time, uint8_t* out, size_t cap) {
+    if (multi) throw Error(UBP_EINVAL, "frames are built on one-scene engines");
+    const size_t stride = subsample > 0 ? std::max<size_t>(n / subsample, 1) : 1;
+    const size_t count = std::min<size_t>(n / stride, subsample);
+    const size_t bytes = 1 + 8 + 4 + 8 + 20 * count;
+    if (!out) return bytes;
+    if (cap < bytes) throw Error(UBP_EINVAL, "frame buffer too small");
+    out[0] = 0x02;
+    std::memcpy(out + 1, &frame_number, 8);
+    const uint32_t c32 = uint32_t(count);
+    std::memcpy(out + 9, &c32, 4);
+    std::memcpy(out + 13, &sim_time, 8);
+    if (count == 0) return bytes;
+    if (needs_init) {  // no step yet: the input arrays, on the host
+      for (size_t k = 0; k < count; ++k) {
+        const size_t i = k * stride;
+        uint8_t* o = out + 21 + 20 * k;
+        const float v[4] = {h_init[0][i], h_init[1][i], h_init[2][i], h_temp[i]};
+        std::memcpy(o, v, 16);
+        const float rho0 = h_ft[i] == 0 ? 1000.0f : 1.204f;
+        const float r = (h_init[9][i] / rho0) * 1000.0f;
+        const uint16_t ratio = r >= 65535.0f ? 65535 : (r > 0.0f ? uint16_t(r) : 0);
+        o[16] = h_ft[i];
+        std::memcpy(o + 17, &ratio, 2);
+        o[19] = 0;
+      }
+      return bytes;
+    }
+    if (!d_temp) {
+      dalloc(&d_temp, n);
+      HIPCHK(hipMemcpy(d_temp, h_temp.data(), n * 4, hipMemcpyHostToDevice));
+      dalloc(&finv, n);
+    }
+    if (5 * count > fbuf_n) {
+      dfree(fbuf);
+      dalloc(&fbuf, 5 * count);
+      fbuf_n = 5 * count;
+    }
+    launch(PASS_ELEM, k_inverse, n, (const uint32_t*)orig[pp], uint32_t(n), finv);
+    launch(PASS_ELEM, k_frame, count, P, uint32_t(count), uint32_t(stride), (const uint32_t*)finv,
+           (const float4*)pos, (const float*)rho_n, (const uint8_t*)ft[pp], (const float*)d_temp, fbuf);
+    HIPCHK(hipMemcpyAsync(out + 21, fbuf, 20 * count, hipMemcpyDeviceToHost, st));
+    sync();
+    return bytes;
+  }
+
   // ---- force on a geometry (orchestrator/src/force.rs:39-129) ----
   void set_force_sdf(const float o[3], float cellsz, const uint32_t d[3], const float* data) {
     if (multi) throw Error(UBP_EINVAL, "force extraction runs on one-scene engines");
@@ -1897,6 +1956,16 @@ int ubp_trainer_get_model(ubp_trainer* t, ubp_mlp_model* out) {
   if (!out) throw Error(UBP_EINVAL, "null model");
   if (!t->tr.stats_set) throw Error(UBP_EINVAL, "no model yet (init_model)");
   t->tr.M.download(out, t->tr.st);
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_build_frame(ubp_engine* e, uint32_t subsample_count, uint64_t frame_number, double sim_time,
+                    uint8_t* out, size_t capacity, size_t* size) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  if (!size) throw Error(UBP_EINVAL, "null size");
+  *size = e->eng.build_frame(subsample_count, frame_number, sim_time, out, capacity);
   return UBP_OK;
   UBP_API_END
 }

@@ -1421,6 +1421,33 @@ __global__ __launch_bounds__(256) void k_surface_forces(DevParams P, DevParams G
   if (threadIdx.x < 10) part[10 * blockIdx.x + threadIdx.x] = sm[threadIdx.x][0];
 }
 
+// ---- viewer frame (backend/crates/server/src/ws.rs:232-294) ----
+// Particle k of the frame is original particle k * stride: 20 bytes =
+// x y z temperature (f32) | fluid type u8, density ratio u16 =
+// saturate((rho / rho0) * 1000) truncated, reserved u8 (little endian),
+// written as 5 words.
+__global__ UBP_LB void k_frame(DevParams P, uint32_t count, uint32_t stride, const uint32_t* __restrict__ inv,
+                               const float4* __restrict__ pos, const float* __restrict__ rho,
+                               const uint8_t* __restrict__ ft, const float* __restrict__ temp,
+                               uint32_t* __restrict__ out) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const uint32_t id = k * stride;
+  const uint32_t s = inv[id];
+  const float4 p = pos[s];
+  const int f = ft[s];
+  float r = (rho[s] / P.rho0[f]) * 1000.0f;
+  uint32_t ratio = 0u;  // float -> u16 saturating cast (NaN -> 0)
+  if (r >= 65535.0f) ratio = 65535u;
+  else if (r > 0.0f) ratio = uint32_t(r);
+  uint32_t* o = out + 5u * k;
+  o[0] = __float_as_uint(p.x);
+  o[1] = __float_as_uint(p.y);
+  o[2] = __float_as_uint(p.z);
+  o[3] = __float_as_uint(temp[id]);
+  o[4] = uint32_t(f) | (ratio << 8);
+}
+
 // ---- diagnostics ----
 // Plain streaming copy (HBM counter calibration).
 __global__ UBP_LB void k_calib_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n4) {
