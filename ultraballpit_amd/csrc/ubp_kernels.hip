@@ -291,19 +291,64 @@ __device__ __forceinline__ void walk27r(const DevParams& P, uint32_t i, float4 p
   }
 }
 
-template <int STRIDE, int UNROLL = 2, class F>
+// Visit the listed neighbours of i in list order: f(j, pos_j, rec_j), where
+// rec_j points at the NV float4s of j's record that f may read.  Lists that
+// overflowed re-walk the cells (walk27r).  Two pipelines, chosen per pass by
+// measurement; the bodies run in list order either way, so the sums are the
+// same bits:
+//  MODE 0  UNROLL neighbours per trip, their list entries and records loaded
+//          together at the top of the trip;
+//  MODE 1  list entries two trips ahead and records one trip ahead, held in
+//          registers: the loads of the next trip overlap this trip's bodies.
+template <int STRIDE, int UNROLL = 2, int NV = STRIDE, int MODE = 0, class F>
 __device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, float4 pi,
                                               const float4* __restrict__ rec,
                                               const uint32_t* __restrict__ start,
                                               const uint32_t* __restrict__ nbr,
                                               const uint32_t* __restrict__ nn, F&& f) {
   const uint32_t c = nn[i];
-  if (c <= P.K) {  // no overflow bit and the list fits
-    const uint32_t* l = nbr + i;
-    const size_t stride = P.n;
-    uint32_t t = 0;
-    // UNROLL neighbours per trip: their records in flight together, bodies
-    // in list order
+  if (c > P.K) {  // overflow bit set, or the list did not fit
+    walk27r<STRIDE>(P, i, pi, rec, start, f);
+    return;
+  }
+  const uint32_t* l = nbr + i;
+  const size_t stride = P.n;
+  uint32_t t = 0;
+  if constexpr (MODE == 1) {
+    uint32_t ja[UNROLL], jb[UNROLL];
+    float4 qa[UNROLL][NV];
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u) {
+      ja[u] = (uint32_t(u) < c) ? l[size_t(u) * stride] : 0u;
+      jb[u] = (uint32_t(UNROLL + u) < c) ? l[size_t(UNROLL + u) * stride] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < UNROLL; ++u)
+#pragma unroll
+      for (int v = 0; v < NV; ++v) qa[u][v] = rec[size_t(STRIDE) * ja[u] + v];
+    for (; t + UNROLL <= c; t += UNROLL) {
+      float4 qb[UNROLL][NV];
+      uint32_t jc[UNROLL];
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u)  // entries past the list end read slot 0, unused
+#pragma unroll
+        for (int v = 0; v < NV; ++v) qb[u][v] = rec[size_t(STRIDE) * jb[u] + v];
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) {
+        const uint32_t tn = t + 2u * UNROLL + u;
+        jc[u] = (tn < c) ? l[size_t(tn) * stride] : 0u;
+      }
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) f(ja[u], qa[u][0], qa[u]);
+#pragma unroll
+      for (int u = 0; u < UNROLL; ++u) {
+        ja[u] = jb[u];
+        jb[u] = jc[u];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) qa[u][v] = qb[u][v];
+      }
+    }
+  } else {
     for (; t + UNROLL <= c; t += UNROLL) {
       uint32_t j[UNROLL];
       const float4* r[UNROLL];
@@ -315,13 +360,11 @@ __device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, fl
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) f(j[u], q[u], r[u]);
     }
-    for (; t < c; ++t) {
-      const uint32_t j0 = l[size_t(t) * stride];
-      const float4* r0 = rec + size_t(STRIDE) * j0;
-      f(j0, r0[0], r0);
-    }
-  } else {
-    walk27r<STRIDE>(P, i, pi, rec, start, f);
+  }
+  for (; t < c; ++t) {  // the last c mod UNROLL
+    const uint32_t j0 = l[size_t(t) * stride];
+    const float4* r0 = rec + size_t(STRIDE) * j0;
+    f(j0, r0[0], r0);
   }
 }
 
@@ -372,7 +415,8 @@ __global__ UBP_LB void k_neigh(DevParams P0, Scenes S, const float4* __restrict_
 // rows are produced in rounds of 32: in round r every lane decodes its masks
 // (held in registers) into entries [32 r, 32 r + 32) of its list, parks them
 // in a 32 x 64 LDS tile, and the wavefront then stores the 32 rows, 64
-// consecutive slots per row.
+// consecutive slots per row.  Decoded bits are cleared from the masks, so a
+// round resumes where the previous one stopped.
 #define UBP_EXPAND_CHUNK 32
 template <bool M>
 __global__ __launch_bounds__(64) void k_expand(DevParams P0, Scenes S, const float4* __restrict__ pos,
@@ -417,19 +461,17 @@ __global__ __launch_bounds__(64) void k_expand(DevParams P0, Scenes S, const flo
   uint32_t* l = nbr + i;
   for (uint32_t r0 = 0; r0 < tmax; r0 += UBP_EXPAND_CHUNK) {
     // entries [r0, r0 + 32) of this lane's list
-    uint32_t skip = r0, e = 0;
+    uint32_t e = 0;
 #pragma unroll
     for (int k = 0; k < 27; ++k) {
       uint32_t mask = mk[k];
-      const uint32_t pc = uint32_t(__popc(mask));
-      if (skip >= pc) { skip -= pc; continue; }
-      for (; skip > 0u; --skip) mask &= mask - 1u;
       while (mask && e < UBP_EXPAND_CHUNK) {
         const uint32_t bit = uint32_t(__builtin_ctz(mask));
         mask &= mask - 1u;
         tile[e][lane] = bk[k] + bit;
         ++e;
       }
+      mk[k] = mask;  // the next round resumes here
     }
     __syncthreads();
     const uint32_t rows = (tmax - r0 < UBP_EXPAND_CHUNK) ? tmax - r0 : UBP_EXPAND_CHUNK;
@@ -460,7 +502,7 @@ __global__ UBP_LB void k_density(DevParams P0, Scenes S, const float4* __restric
   Kahan rho, dif;
   rho.init(pi.w * P.norm_h3);  // m_i W(0, h)
   dif.init(0.0f);
-  for_neighbors<2>(P, i, pi, dr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
+  for_neighbors<2, 8>(P, i, pi, dr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
     float dx = pi.x - q.x;
     float dy = pi.y - q.y;
     float dz = pi.z - q.z;
@@ -529,7 +571,7 @@ __global__ UBP_LB void k_forces(DevParams P0, Scenes S, const float4* __restrict
   Kahan fx, fy, fz, vx, vy, vz;
   fx.init(0.0f); fy.init(0.0f); fz.init(0.0f);
   vx.init(0.0f); vy.init(0.0f); vz.init(0.0f);
-  for_neighbors<4>(P, i, pi, fr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
+  for_neighbors<4, 4, 3, 1>(P, i, pi, fr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
     float dx = pi.x - q.x;
     float dy = pi.y - q.y;
     float dz = pi.z - q.z;
@@ -1224,7 +1266,7 @@ __global__ UBP_LB void k_xsph(DevParams P0, Scenes S, const float4* __restrict__
   const float4 vi = xr[2u * i + 1u];
   const float rho_i = vi.w;
   float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-  for_neighbors<2, 4>(P, i, pi, xr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
+  for_neighbors<2, 10>(P, i, pi, xr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
     float dx = pi.x - q.x;
     float dy = pi.y - q.y;
     float dz = pi.z - q.z;
