@@ -958,7 +958,7 @@ struct Engine {
     sync();
     if (perm) std::memcpy(perm, ho.data(), n * 4);
     if (counts)
-      for (size_t s = 0; s < n; ++s) counts[ho[s]] = hv[s];
+      for (size_t s = 0; s < n; ++s) counts[ho[s]] = hv[s] & ~UBP_NN_OVF;
   }
 
   void save_snap(Snapshot& S) {
