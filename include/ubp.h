@@ -242,6 +242,46 @@ int ubp_set_tuning_ex(ubp_engine *e, const ubp_tuning *t);
 int ubp_build_frame(ubp_engine *e, uint32_t subsample_count, uint64_t frame_number, double sim_time,
                     uint8_t *out, size_t capacity, size_t *size);
 
+/* ---- Domain set-up on the device: replaces setup_domain
+ * (backend/crates/orchestrator/src/domain.rs:35-85) with
+ * place_fluid_particles (:88-159), generate_wall_boundary_particles
+ * (:162-348) and generate_geometry_boundary_particles (:351-402), the parts
+ * of SimulationConfig they read (config.rs).  Fluid particles on the
+ * lattice (i + 0.5) spacing, i outer and k inner, skipping sites outside
+ * the domain box or with SDF < 0; then three wall layers per Wall face
+ * (faces x_min, x_max, y_min, y_max, z_min, z_max); then three layers
+ * along the inward normal under every site with 0 < SDF < spacing / 2.
+ * The SDF is a GridSDF (layout as ubp_set_sdf) sampled as
+ * GridSDF::sample / gradient (geometry.rs:99-175), f32::MAX outside the
+ * grid.  With fluid and boundary NULL only the counts are returned;
+ * otherwise the capacities must hold them.  Velocities, accelerations and
+ * pressure are zero (particle.rs:96-120).  Synchronous. */
+typedef struct {
+  float domain_min[3], domain_max[3];
+  float particle_spacing;
+  int32_t fluid_type;        /* 0 Water, 1 Air, 2 Mixed: bottom half (y) water (config.rs:66-73) */
+  float initial_temperature;
+  uint32_t walls;            /* bit f set: face f is a Wall (BoundaryConditions, config.rs:156-167) */
+} ubp_domain;
+
+typedef struct {
+  float *x, *y, *z, *mass, *nx, *ny, *nz;
+} ubp_boundary_out;
+
+int ubp_setup_domain(const ubp_domain *d, const float sdf_origin[3], float sdf_cell, const uint32_t sdf_dims[3],
+                     const float *sdf, ubp_particles_out *fluid, size_t fluid_capacity, ubp_boundary_out *boundary,
+                     size_t boundary_capacity, size_t *n_fluid, size_t *n_boundary);
+
+/* GridSDF node values of analytic shapes, computed on the device in FP64
+ * and rounded once to FP32 (no counterpart in the reference, which builds
+ * SDFs from STL meshes with mesh_to_sdf): node (i, j, k) at
+ * origin + cell (i, j, k), z fastest.  kind 0: distance inside the box
+ * params[0..2] .. params[3..5] (positive inside); kind 1: |x - c| - r,
+ * c = params[0..2], r = params[3].  combine != 0 stores the minimum with
+ * the values already in out.  Synchronous. */
+int ubp_sdf_analytic(int kind, const double *params, const double origin[3], double cell, const uint32_t dims[3],
+                     int combine, float *out);
+
 /* ---- Force of the fluid on a geometry: replaces compute_surface_forces
  * (backend/crates/orchestrator/src/force.rs:39-129; SurfaceForce
  * force.rs:11-16).  The geometry is an SDF grid of its own (the body whose

@@ -111,6 +111,9 @@ def lib():
     L.orc_sdf_gradient.argtypes = [fp, C.c_float, u32p, fp, fp, fp]
     L.orc_neighbors.argtypes = [C.c_size_t, fp, fp, fp, C.c_float, fp, fp, C.c_float,
                                 C.c_size_t, u32p, C.c_size_t]
+    L.orc_setup_domain.argtypes = ([fp, fp, C.c_float, C.c_int, C.c_float, C.c_uint32, fp, C.c_float, u32p, fp,
+                                    C.c_size_t] + [fp] * 6 + [u8p, C.c_size_t] + [fp] * 7
+                                   + [C.POINTER(C.c_size_t)] * 2)
     L.orc_check_div_h.argtypes = [C.c_float, C.c_float, C.c_float]
     L.orc_check_div_h.restype = C.c_long
     _lib = L
@@ -339,6 +342,36 @@ def sdf_gradient(sdf: dict, p):
     data = _f32(sdf["data"]).ravel(); pt = _f32(p); out = np.zeros(3, np.float32)
     ok = lib().orc_sdf_gradient(_fp(o), float(sdf["cell"]), _u32(d), _fp(data), _fp(pt), _fp(out))
     return out.astype(np.float64), bool(ok)
+
+
+FLUID_FIELDS = ("x", "y", "z", "mass", "density", "temperature")
+BOUNDARY_FIELDS = ("x", "y", "z", "mass", "nx", "ny", "nz")
+WALL_FACES = ("x_min", "x_max", "y_min", "y_max", "z_min", "z_max")
+
+
+def setup_domain(domain_min, domain_max, spacing, fluid_type, temperature, walls, sdf: dict):
+    """setup_domain (domain.rs:35-85): (fluid dict, boundary dict) in the
+    reference's order.  fluid_type 0 Water / 1 Air / 2 Mixed; walls = bit
+    mask over x_min, x_max, y_min, y_max, z_min, z_max."""
+    dmin = _f32(domain_min); dmax = _f32(domain_max)
+    o = _f32(sdf["origin"]); d = np.ascontiguousarray(sdf["dims"], np.uint32)
+    data = _f32(sdf["data"]).ravel()
+    nf = C.c_size_t(0); nb = C.c_size_t(0)
+
+    def call(cf, cb):
+        F = {k: np.zeros(max(cf, 1), np.float32) for k in FLUID_FIELDS}
+        F["fluid_type"] = np.zeros(max(cf, 1), np.uint8)
+        B = {k: np.zeros(max(cb, 1), np.float32) for k in BOUNDARY_FIELDS}
+        lib().orc_setup_domain(_fp(dmin), _fp(dmax), float(spacing), int(fluid_type), float(temperature),
+                               int(walls), _fp(o), float(sdf["cell"]), _u32(d), _fp(data), cf,
+                               *[_fp(F[k]) for k in FLUID_FIELDS], F["fluid_type"].ctypes.data_as(
+                                   C.POINTER(C.c_uint8)), cb, *[_fp(B[k]) for k in BOUNDARY_FIELDS],
+                               C.byref(nf), C.byref(nb))
+        return F, B
+
+    call(0, 0)
+    F, B = call(nf.value, nb.value)
+    return ({k: v[:nf.value] for k, v in F.items()}, {k: v[:nb.value] for k, v in B.items()})
 
 
 def neighbors(x, y, z, cell, dmin, dmax, radius, i, cap=4096):

@@ -1231,4 +1231,156 @@ int orc_neighbors(size_t n, const float* x, const float* y, const float* z, floa
   return int(k);
 }
 
+// ---- Domain set-up (orchestrator/src/domain.rs:35-402) ----
+// GridSDF::sample exactly as the reference (geometry.rs:99-150): f32::MAX
+// outside the node grid; `floor() as i32` saturates and maps NaN to 0, i0 + 1
+// wraps; a node index past the data reads f32::MAX (get_distance, :178-181).
+struct RefSdf {
+  float origin[3];
+  float cell;
+  uint32_t dims[3];
+  const float* data;
+  size_t count;
+  static int32_t sat_i32(float v) {
+    if (std::isnan(v)) return 0;
+    if (v <= -2147483648.0f) return INT32_MIN;
+    if (v >= 2147483648.0f) return INT32_MAX;
+    return int32_t(v);
+  }
+  float node(uint32_t i, uint32_t j, uint32_t k) const {
+    const uint32_t idx = i * dims[1] * dims[2] + j * dims[2] + k;  // u32, wrapping
+    return size_t(idx) < count ? data[idx] : FLT_MAX;
+  }
+  float sample(float px, float py, float pz) const {
+    const float gx = (px - origin[0]) / cell;
+    const float gy = (py - origin[1]) / cell;
+    const float gz = (pz - origin[2]) / cell;
+    const int32_t i0 = sat_i32(std::floor(gx)), j0 = sat_i32(std::floor(gy)), k0 = sat_i32(std::floor(gz));
+    if (i0 < 0 || j0 < 0 || k0 < 0) return FLT_MAX;
+    const int32_t i1 = int32_t(uint32_t(i0) + 1u), j1 = int32_t(uint32_t(j0) + 1u), k1 = int32_t(uint32_t(k0) + 1u);
+    if (i1 >= int32_t(dims[0]) || j1 >= int32_t(dims[1]) || k1 >= int32_t(dims[2])) return FLT_MAX;
+    const float fx = gx - float(i0), fy = gy - float(j0), fz = gz - float(k0);
+    const uint32_t a0 = uint32_t(i0), b0 = uint32_t(j0), c0_ = uint32_t(k0);
+    const uint32_t a1 = uint32_t(i1), b1 = uint32_t(j1), c1_ = uint32_t(k1);
+    const float c000 = node(a0, b0, c0_), c001 = node(a0, b0, c1_);
+    const float c010 = node(a0, b1, c0_), c011 = node(a0, b1, c1_);
+    const float c100 = node(a1, b0, c0_), c101 = node(a1, b0, c1_);
+    const float c110 = node(a1, b1, c0_), c111 = node(a1, b1, c1_);
+    const float c00 = c000 * (1.0f - fz) + c001 * fz;
+    const float c01 = c010 * (1.0f - fz) + c011 * fz;
+    const float c10 = c100 * (1.0f - fz) + c101 * fz;
+    const float c11 = c110 * (1.0f - fz) + c111 * fz;
+    const float c0 = c00 * (1.0f - fy) + c01 * fy;
+    const float c1 = c10 * (1.0f - fy) + c11 * fy;
+    return c0 * (1.0f - fx) + c1 * fx;
+  }
+  // geometry.rs:153-175: central differences at +-cell/2, normalised when |g| > 1e-8
+  void gradient(float px, float py, float pz, float g[3]) const {
+    const float h = cell * 0.5f;
+    const float dx = (sample(px + h, py, pz) - sample(px - h, py, pz)) / (2.0f * h);
+    const float dy = (sample(px, py + h, pz) - sample(px, py - h, pz)) / (2.0f * h);
+    const float dz = (sample(px, py, pz + h) - sample(px, py, pz - h)) / (2.0f * h);
+    const float mag = std::sqrt(dx * dx + dy * dy + dz * dz);
+    if (mag > 1e-8f) {
+      g[0] = dx / mag; g[1] = dy / mag; g[2] = dz / mag;
+    } else {
+      g[0] = g[1] = g[2] = 0.0f;
+    }
+  }
+};
+
+// setup_domain (domain.rs:35-85): fluid particles on the (i + 0.5) spacing
+// lattice, i outer and k inner, skipping sites outside the domain box or
+// with SDF < 0 (place_fluid_particles, :88-159); then three wall layers at
+// (layer + 0.5) spacing outside each Wall face, faces x-, x+, y-, y+, z-,
+// z+ (generate_wall_boundary_particles, :162-348); then three layers along
+// the inward normal under every lattice site with 0 < SDF < spacing / 2
+// (generate_geometry_boundary_particles, :351-402).  fluid_type: 0 Water,
+// 1 Air, 2 Mixed (config.rs:66-73); walls: bit f set = face f is a Wall.
+// Writes entries below the capacities and returns both counts.
+void orc_setup_domain(const float dmin[3], const float dmax[3], float spacing, int fluid_type, float temperature,
+                      uint32_t walls, const float so[3], float scell, const uint32_t sd[3], const float* sdf,
+                      size_t cap_f, float* fx, float* fy, float* fz, float* fm, float* frho, float* ftemp,
+                      uint8_t* ftype, size_t cap_b, float* bx, float* by, float* bz, float* bm, float* bnx,
+                      float* bny, float* bnz, size_t* n_fluid, size_t* n_boundary) {
+  RefSdf S;
+  for (int a = 0; a < 3; ++a) { S.origin[a] = so[a]; S.dims[a] = sd[a]; }
+  S.cell = scell;
+  S.data = sdf;
+  S.count = size_t(sd[0]) * sd[1] * sd[2];
+  const float volume = spacing * (spacing * spacing);  // powi(3)
+  const float water_mass = 1000.0f * volume;
+  const float air_mass = 1.2f * volume;
+  const float center_y = (dmin[1] + dmax[1]) * 0.5f;
+  size_t cnt[3];
+  for (int a = 0; a < 3; ++a) {  // `ceil() as usize` saturates: NaN and negatives give 0
+    const float c = std::ceil((dmax[a] - dmin[a]) / spacing);
+    cnt[a] = (c > 0.0f) ? size_t(c) : 0;
+  }
+  size_t nf = 0;
+  for (size_t i = 0; i < cnt[0]; ++i)
+    for (size_t j = 0; j < cnt[1]; ++j)
+      for (size_t k = 0; k < cnt[2]; ++k) {
+        const float x = dmin[0] + (float(i) + 0.5f) * spacing;
+        const float y = dmin[1] + (float(j) + 0.5f) * spacing;
+        const float z = dmin[2] + (float(k) + 0.5f) * spacing;
+        if (x < dmin[0] || x > dmax[0] || y < dmin[1] || y > dmax[1] || z < dmin[2] || z > dmax[2]) continue;
+        if (S.sample(x, y, z) < 0.0f) continue;
+        const bool water = fluid_type == 0 || (fluid_type == 2 && y < center_y);
+        if (nf < cap_f) {
+          fx[nf] = x; fy[nf] = y; fz[nf] = z;
+          fm[nf] = water ? water_mass : air_mass;
+          frho[nf] = water ? 1000.0f : 1.2f;
+          ftemp[nf] = temperature;
+          ftype[nf] = water ? 0 : 1;
+        }
+        ++nf;
+      }
+  const float bmass = fluid_type == 1 ? air_mass : water_mass;
+  size_t nb = 0;
+  auto push = [&](float x, float y, float z, float nx, float ny, float nz) {
+    if (nb < cap_b) {
+      bx[nb] = x; by[nb] = y; bz[nb] = z; bm[nb] = bmass;
+      bnx[nb] = nx; bny[nb] = ny; bnz[nb] = nz;
+    }
+    ++nb;
+  };
+  for (int f = 0; f < 6; ++f) {
+    if (!(walls & (1u << f))) continue;
+    const int a = f / 2;                  // wall axis
+    const bool hi = (f & 1) != 0;
+    const int u = a == 0 ? 1 : 0;         // outer loop axis
+    const int v = a == 2 ? 1 : 2;         // inner loop axis
+    for (int layer = 0; layer < 3; ++layer)
+      for (size_t p = 0; p < cnt[u]; ++p)
+        for (size_t q = 0; q < cnt[v]; ++q) {
+          float c[3], n[3] = {0.0f, 0.0f, 0.0f};
+          c[u] = dmin[u] + (float(p) + 0.5f) * spacing;
+          c[v] = dmin[v] + (float(q) + 0.5f) * spacing;
+          const float off = (float(layer) + 0.5f) * spacing;
+          c[a] = hi ? dmax[a] + off : dmin[a] - off;
+          n[a] = hi ? -1.0f : 1.0f;
+          push(c[0], c[1], c[2], n[0], n[1], n[2]);
+        }
+  }
+  const float thr = spacing * 0.5f;
+  for (size_t i = 0; i < cnt[0]; ++i)
+    for (size_t j = 0; j < cnt[1]; ++j)
+      for (size_t k = 0; k < cnt[2]; ++k) {
+        const float x = dmin[0] + (float(i) + 0.5f) * spacing;
+        const float y = dmin[1] + (float(j) + 0.5f) * spacing;
+        const float z = dmin[2] + (float(k) + 0.5f) * spacing;
+        const float d = S.sample(x, y, z);
+        if (!(d > 0.0f && d < thr)) continue;
+        float g[3];
+        S.gradient(x, y, z, g);
+        for (int layer = 0; layer < 3; ++layer) {
+          const float off = (float(layer) + 0.5f) * spacing;
+          push(x - g[0] * off, y - g[1] * off, z - g[2] * off, g[0], g[1], g[2]);
+        }
+      }
+  *n_fluid = nf;
+  *n_boundary = nb;
+}
+
 }  // extern "C"

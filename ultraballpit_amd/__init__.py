@@ -125,6 +125,15 @@ class _SurfaceForce(C.Structure):
                 ("net_force", C.c_double * 3), ("net_moment", C.c_double * 3), ("samples", C.c_uint64)]
 
 
+class _Domain(C.Structure):
+    _fields_ = [("domain_min", C.c_float * 3), ("domain_max", C.c_float * 3), ("particle_spacing", C.c_float),
+                ("fluid_type", C.c_int32), ("initial_temperature", C.c_float), ("walls", C.c_uint32)]
+
+
+class _BoundaryOut(C.Structure):
+    _fields_ = [(k, C.POINTER(C.c_float)) for k in ("x", "y", "z", "mass", "nx", "ny", "nz")]
+
+
 class _Scene(C.Structure):
     _fields_ = [("particles", C.POINTER(_ParticlesIn)), ("h", C.c_float), ("gravity", C.c_float * 3),
                 ("speed_of_sound", C.c_float), ("cfl_number", C.c_float),
@@ -147,7 +156,8 @@ EXPORTS = ("ubp_create", "ubp_destroy", "ubp_last_error", "ubp_gpu_available",
            "ubp_mfma_selftest", "ubp_surrogate_set_model", "ubp_surrogate_run", "ubp_surrogate_mark",
            "ubp_surrogate_step", "ubp_surrogate_features", "ubp_trainer_create", "ubp_trainer_destroy",
            "ubp_trainer_collect", "ubp_trainer_finish", "ubp_trainer_rows", "ubp_trainer_init_model", "ubp_trainer_fit",
-           "ubp_trainer_get_model", "ubp_set_force_sdf", "ubp_surface_forces", "ubp_build_frame")
+           "ubp_trainer_get_model", "ubp_set_force_sdf", "ubp_surface_forces", "ubp_build_frame",
+           "ubp_setup_domain", "ubp_sdf_analytic")
 
 #: kernel order of ubp_step_n_profiled
 KERNELS = ("k_kick", "k_xsph", "k_drift", "k_scan", "k_scatter", "k_rank_gather", "k_neigh", "k_expand",
@@ -230,6 +240,11 @@ def lib():
     L.ubp_build_frame.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64, C.c_double, C.POINTER(C.c_uint8),
                                   C.c_size_t, C.POINTER(C.c_size_t)]
     L.ubp_surface_forces.argtypes = [C.c_void_p, C.c_float, C.c_int, C.POINTER(_SurfaceForce)]
+    L.ubp_setup_domain.argtypes = [C.POINTER(_Domain), fp, C.c_float, u32p, fp, C.POINTER(_ParticlesOut),
+                                   C.c_size_t, C.POINTER(_BoundaryOut), C.c_size_t, C.POINTER(C.c_size_t),
+                                   C.POINTER(C.c_size_t)]
+    L.ubp_sdf_analytic.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double, u32p,
+                                   C.c_int, fp]
     for name in EXPORTS:
         getattr(L, name)
     _lib = L

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -71,6 +72,7 @@ template <class T> static void dfree(T*& p) {
 
 }  // namespace ubp
 #include "ubp_learn.hip"
+#include "ubp_domain.hip"
 namespace ubp {
 
 // Morton grid of one scene (neighbor.rs:29-46 dims; cell = 2h, lib.rs:149).
@@ -1958,6 +1960,140 @@ int ubp_trainer_get_model(ubp_trainer* t, ubp_mlp_model* out) {
   if (!out) throw Error(UBP_EINVAL, "null model");
   if (!t->tr.stats_set) throw Error(UBP_EINVAL, "no model yet (init_model)");
   t->tr.M.download(out, t->tr.st);
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_setup_domain(const ubp_domain* d, const float sdf_origin[3], float sdf_cell, const uint32_t sdf_dims[3],
+                     const float* sdf, ubp_particles_out* fluid, size_t fluid_capacity, ubp_boundary_out* boundary,
+                     size_t boundary_capacity, size_t* n_fluid, size_t* n_boundary) {
+  UBP_API_BEGIN
+  if (!d || !sdf_origin || !sdf_dims || !sdf || !n_fluid || !n_boundary) throw Error(UBP_EINVAL, "null argument");
+  if (!(d->particle_spacing > 0.0f)) throw Error(UBP_EINVAL, "particle spacing must be positive");
+  if (d->fluid_type < 0 || d->fluid_type > 2) throw Error(UBP_EINVAL, "fluid type must be 0, 1 or 2");
+  if (!(sdf_cell > 0.0f) || !sdf_dims[0] || !sdf_dims[1] || !sdf_dims[2]) throw Error(UBP_EINVAL, "invalid SDF grid");
+  ubp::require_gfx950();
+  ubp::DomDev D{};
+  const float s = d->particle_spacing;
+  for (int a = 0; a < 3; ++a) { D.dmin[a] = d->domain_min[a]; D.dmax[a] = d->domain_max[a]; }
+  D.spacing = s;
+  D.temperature = d->initial_temperature;
+  D.center_y = (D.dmin[1] + D.dmax[1]) * 0.5f;
+  const float volume = s * (s * s);  // powi(3)
+  D.water_mass = 1000.0f * volume;
+  D.air_mass = 1.2f * volume;
+  D.bmass = d->fluid_type == 1 ? D.air_mass : D.water_mass;
+  D.thr = s * 0.5f;
+  D.fluid_type = d->fluid_type;
+  uint64_t cnt[3];
+  for (int a = 0; a < 3; ++a) {  // `ceil() as usize` saturates: NaN and negatives give 0
+    const float c = std::ceil((D.dmax[a] - D.dmin[a]) / s);
+    if (c >= 4294967296.0f) throw Error(UBP_EOVERFLOW, "domain too large for its spacing");
+    cnt[a] = (c > 0.0f) ? uint64_t(c) : 0u;
+    D.cnt[a] = uint32_t(cnt[a]);
+  }
+  const uint64_t nsites = cnt[0] * cnt[1] * cnt[2];
+  uint64_t nwall = 0;
+  for (int f = 0; f < 6; ++f)
+    if (d->walls & (1u << f)) {
+      const int a = f / 2, u = a == 0 ? 1 : 0, v = a == 2 ? 1 : 2;
+      nwall += 3u * cnt[u] * cnt[v];
+    }
+  if (nsites >= (uint64_t(1) << 32) || nwall >= (uint64_t(1) << 31))
+    throw Error(UBP_EOVERFLOW, "too many lattice sites");
+  for (int a = 0; a < 3; ++a) { D.S.origin[a] = sdf_origin[a]; D.S.dims[a] = sdf_dims[a]; }
+  D.S.cell = sdf_cell;
+  D.S.count = uint64_t(sdf_dims[0]) * sdf_dims[1] * sdf_dims[2];
+  ubp::ScopedStream S;
+  ubp::DevArr<float> dsdf{size_t(D.S.count)};
+  HIPCHK(hipMemcpyAsync(dsdf.p, sdf, size_t(D.S.count) * 4, hipMemcpyHostToDevice, S.s));
+  D.S.data = dsdf.p;
+  const uint32_t nblk = uint32_t((nsites + UBP_DOM_SITES - 1) / UBP_DOM_SITES);
+  ubp::DevArr<uint32_t> bf(nblk), bg(nblk);
+  std::vector<uint32_t> hf(nblk), hg(nblk);
+  if (nblk) {
+    hipLaunchKernelGGL(ubp::k_dom_count, dim3(nblk), dim3(256), 0, S.s, D, uint32_t(nsites), bf.p, bg.p);
+    LAUNCHCHK("k_dom_count");
+    HIPCHK(hipMemcpyAsync(hf.data(), bf.p, size_t(nblk) * 4, hipMemcpyDeviceToHost, S.s));
+    HIPCHK(hipMemcpyAsync(hg.data(), bg.p, size_t(nblk) * 4, hipMemcpyDeviceToHost, S.s));
+    HIPCHK(hipStreamSynchronize(S.s));
+  }
+  uint64_t nf = 0, ng = 0;
+  for (uint32_t b = 0; b < nblk; ++b) {  // exclusive scan of the block counts
+    const uint32_t f = hf[b], g = hg[b];
+    hf[b] = uint32_t(nf);
+    hg[b] = uint32_t(ng);
+    nf += f;
+    ng += g;
+  }
+  const uint64_t nb = nwall + 3u * ng;
+  if (nb >= (uint64_t(1) << 32)) throw Error(UBP_EOVERFLOW, "too many boundary particles");
+  *n_fluid = size_t(nf);
+  *n_boundary = size_t(nb);
+  if (!fluid && !boundary) return UBP_OK;
+  if (fluid && fluid_capacity < nf) throw Error(UBP_EINVAL, "fluid capacity too small");
+  if (boundary && boundary_capacity < nb) throw Error(UBP_EINVAL, "boundary capacity too small");
+  ubp::DevArr<float> fo{size_t(nf) * 6}, bo{size_t(nb) * 7};
+  ubp::DevArr<uint8_t> ft{size_t(nf)};
+  ubp::DomOut O;
+  O.fx = fo.p; O.fy = fo.p + nf; O.fz = fo.p + 2 * nf; O.fm = fo.p + 3 * nf; O.frho = fo.p + 4 * nf;
+  O.ftemp = fo.p + 5 * nf; O.ftype = ft.p;
+  O.bx = bo.p; O.by = bo.p + nb; O.bz = bo.p + 2 * nb; O.bm = bo.p + 3 * nb; O.bnx = bo.p + 4 * nb;
+  O.bny = bo.p + 5 * nb; O.bnz = bo.p + 6 * nb;
+  if (nblk) {
+    HIPCHK(hipMemcpyAsync(bf.p, hf.data(), size_t(nblk) * 4, hipMemcpyHostToDevice, S.s));
+    HIPCHK(hipMemcpyAsync(bg.p, hg.data(), size_t(nblk) * 4, hipMemcpyHostToDevice, S.s));
+    hipLaunchKernelGGL(ubp::k_dom_write, dim3(nblk), dim3(256), 0, S.s, D, uint32_t(nsites), (const uint32_t*)bf.p,
+                       (const uint32_t*)bg.p, uint32_t(nwall), O);
+    LAUNCHCHK("k_dom_write");
+  }
+  if (nwall) {
+    hipLaunchKernelGGL(ubp::k_dom_walls, dim3(uint32_t((nwall + 255) / 256)), dim3(256), 0, S.s, D, d->walls,
+                       uint32_t(nwall), O);
+    LAUNCHCHK("k_dom_walls");
+  }
+  auto get = [&](void* dst, const void* src, size_t bytes) {
+    if (dst && bytes) HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S.s));
+  };
+  if (fluid) {
+    get(fluid->x, O.fx, nf * 4); get(fluid->y, O.fy, nf * 4); get(fluid->z, O.fz, nf * 4);
+    get(fluid->mass, O.fm, nf * 4); get(fluid->density, O.frho, nf * 4); get(fluid->temperature, O.ftemp, nf * 4);
+    get(fluid->fluid_type, O.ftype, nf);
+  }
+  if (boundary) {
+    get(boundary->x, O.bx, nb * 4); get(boundary->y, O.by, nb * 4); get(boundary->z, O.bz, nb * 4);
+    get(boundary->mass, O.bm, nb * 4); get(boundary->nx, O.bnx, nb * 4); get(boundary->ny, O.bny, nb * 4);
+    get(boundary->nz, O.bnz, nb * 4);
+  }
+  if (fluid)  // push_particle (particle.rs:96-120): zero velocity, acceleration and pressure
+    for (float* z : {fluid->vx, fluid->vy, fluid->vz, fluid->ax, fluid->ay, fluid->az, fluid->pressure})
+      if (z) std::memset(z, 0, size_t(nf) * 4);
+  HIPCHK(hipStreamSynchronize(S.s));
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_sdf_analytic(int kind, const double* params, const double origin[3], double cell, const uint32_t dims[3],
+                     int combine, float* out) {
+  UBP_API_BEGIN
+  if (!params || !origin || !dims || !out) throw Error(UBP_EINVAL, "null argument");
+  if (kind != 0 && kind != 1) throw Error(UBP_EINVAL, "kind must be 0 (box) or 1 (sphere)");
+  if (!(cell > 0.0)) throw Error(UBP_EINVAL, "cell must be positive");
+  const uint64_t n = uint64_t(dims[0]) * dims[1] * dims[2];
+  if (n >= (uint64_t(1) << 34)) throw Error(UBP_EOVERFLOW, "SDF grid too large");
+  if (!n) return UBP_OK;
+  ubp::require_gfx950();
+  ubp::ScopedStream S;
+  ubp::DevArr<double> prm(6);
+  ubp::DevArr<float> dout{size_t(n)};
+  HIPCHK(hipMemcpyAsync(prm.p, params, (kind == 0 ? 6 : 4) * 8, hipMemcpyHostToDevice, S.s));
+  if (combine) HIPCHK(hipMemcpyAsync(dout.p, out, size_t(n) * 4, hipMemcpyHostToDevice, S.s));
+  hipLaunchKernelGGL(ubp::k_sdf_analytic, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, S.s, kind,
+                     (const double*)prm.p, origin[0], origin[1], origin[2], cell, dims[0], dims[1], dims[2], combine,
+                     dout.p);
+  LAUNCHCHK("k_sdf_analytic");
+  HIPCHK(hipMemcpyAsync(out, dout.p, size_t(n) * 4, hipMemcpyDeviceToHost, S.s));
+  HIPCHK(hipStreamSynchronize(S.s));
   return UBP_OK;
   UBP_API_END
 }
