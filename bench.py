@@ -55,25 +55,42 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles(kernel: str):
-    """HBM bytes per dispatch of `kernel` from the committed rocprof summary
-    of this workload (profiles/step_kernels_dam_break_1m.md: FETCH_SIZE and
-    WRITE_SIZE per dispatch, already scaled by the copy-kernel factor, MiB)."""
+def _profile_row(kernel: str):
+    """The row of `kernel` in the committed rocprof summary of this workload
+    (profiles/step_kernels_dam_break_1m.md), as a list of cells."""
     path = os.path.join(ROOT, "profiles", "step_kernels_dam_break_1m.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) < 6:
+        if len(cells) < 8:
             continue
         name = re.sub(r"^(void )?ubp::", "", cells[0])
         name = re.sub(r"<.*>$", "", name)
         if name == kernel:
-            try:
-                return (float(cells[4]) + float(cells[5])) * 2.0 ** 20
-            except ValueError:
-                return None
+            return cells
     return None
+
+
+def traffic_from_profiles(kernel: str):
+    """HBM bytes per dispatch of `kernel`: FETCH_SIZE and WRITE_SIZE per
+    dispatch from the rocprof summary, already scaled by the copy-kernel
+    factor, MiB."""
+    cells = _profile_row(kernel)
+    try:
+        return (float(cells[4]) + float(cells[5])) * 2.0 ** 20 if cells else None
+    except ValueError:
+        return None
+
+
+def valu_util_from_profiles(kernel: str):
+    """VALU issue utilisation of `kernel` (SQ_INSTS_VALU x 4 cycles over the
+    SIMD cycles of its duration) from the rocprof summary."""
+    cells = _profile_row(kernel)
+    try:
+        return float(cells[7].rstrip("%")) / 100.0 if cells else None
+    except ValueError:
+        return None
 
 
 def cpu_baseline(scene, budget_s: float):
@@ -336,7 +353,8 @@ def main():
                    "dt_on_device": True, "graph": not args.no_graph},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "alg_bytes_per_launch": alg, "avg_launch_us": per_kernel_us[dom]},
+                     "alg_bytes_per_launch": alg, "avg_launch_us": per_kernel_us[dom],
+                     "valu_issue_util": valu_util_from_profiles(dom)},
         "step_hbm_frac": STEP_ALG_BYTES * value / (HBM_PEAK_GBS * 1e9),
         "kernel_us_per_step": {k: round(v, 2) for k, v in per_kernel_us.items()},
         "eager_ms_per_step": (t3 - t2) * 1e3 / args.steps,
