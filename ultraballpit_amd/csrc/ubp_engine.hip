@@ -333,9 +333,7 @@ struct Engine {
 
   // device state for n particles over P.ncodes codes; uploads the mirror
   void alloc_upload() {
-    dalloc(&pos, n + 8);  // 8 slots of padding: the candidate test reads up to 7 past a cell
-    HIPCHK(hipMemset(pos + n, 0, 8 * sizeof(float4)));
-    dalloc(&pos_t, n); dalloc(&vel, n); dalloc(&vel_t, n); dalloc(&acc, n);
+    dalloc(&pos, n); dalloc(&pos_t, n); dalloc(&vel, n); dalloc(&vel_t, n); dalloc(&acc, n);
     dalloc(&wall, n); dalloc(&wall_t, n); dalloc(&dv, n);
     dalloc(&rho_n, n); dalloc(&prs, n);
     dalloc(&dr, 2 * n); dalloc(&fr, 4 * n); dalloc(&xr, 2 * n);
