@@ -945,7 +945,7 @@ __global__ UBP_LB void k_feat(DevParams P, const float4* __restrict__ sr, const 
   float s0 = pi.w * ir0 * P.norm_h3;
   float s1 = 0.0f, s2 = 0.0f, s3 = 0.0f, s4 = 0.0f, s5 = 0.0f, s6 = 0.0f, s7 = 0.0f;
   float s8 = 0.0f, s9 = 0.0f, s10 = 0.0f, s11 = 0.0f, s12 = 0.0f, s13 = 0.0f;
-  for_neighbors<3>(P, i, pi, sr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
+  for_neighbors<3, 4, 3, 1>(P, i, pi, sr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
     const float dx = pi.x - q.x;
     const float dy = pi.y - q.y;
     const float dz = pi.z - q.z;
