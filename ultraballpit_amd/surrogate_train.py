@@ -53,11 +53,11 @@ def main():
     ap.add_argument("--warm", type=int, default=10, help="settling cycles before collection")
     ap.add_argument("--cycles", type=int, default=10)
     ap.add_argument("--frac", type=float, default=0.125)
-    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--steps", type=int, default=8000)
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--rounds", type=int, default=2, help="1 + rounds of rows on the network's own states")
+    ap.add_argument("--rounds", type=int, default=4, help="1 + rounds of rows on the network's own states")
     args = ap.parse_args()
     if not ubp.gpu_available():
         sys.exit("no gfx950 device")
