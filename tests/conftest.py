@@ -12,6 +12,15 @@ for p in (ROOT, os.path.join(ROOT, "oracle")):
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) device")
     config.addinivalue_line("markers", "slow: long-running CPU check")
+    # the engine library is built in-tree; build it only when nothing is built
+    # and a compiler is present (a checkout that has never run the build)
+    from ultraballpit_amd import build
+    if not os.path.exists(build.LIB):
+        try:
+            build.hipcc()
+        except RuntimeError:
+            return
+        build.build_all()
 
 
 @pytest.fixture(scope="session")
