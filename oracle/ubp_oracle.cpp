@@ -393,14 +393,45 @@ template <class R> struct Sim {
   // index order) plus the viscous traction mu (G + G^T) n from the SPH
   // velocity gradient G = sum (m_j / rho_j) u_j (x) grad W, with FP64 sums;
   // out = (F_p, F_v, M, samples, sum |F| of the nodes).
+  // Particles are bucketed on a grid of cell >= 2h so that each node visits
+  // only the particles of its 27 surrounding buckets, merged into ascending
+  // index order: the particles the loop accepts (r < 2h) and their order are
+  // those of the loop over all particles.  Nodes run in parallel; their
+  // contributions are summed in node order.
   void surface_forces(double mu, int mode, double out[11]) const {
     for (int c = 0; c < 11; ++c) out[c] = 0.0;
     const R thr = R(2) * h;
     const R area = fsdf.cell * fsdf.cell;
     const size_t n = p.size();
-    for (uint32_t i = 0; i < fsdf.dims[0]; ++i)
-      for (uint32_t j = 0; j < fsdf.dims[1]; ++j)
-        for (uint32_t k = 0; k < fsdf.dims[2]; ++k) {
+    // bucket grid over the particles' bounding box
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (size_t q = 0; q < n; ++q) {
+      const double c[3] = {double(p.x[q]), double(p.y[q]), double(p.z[q])};
+      for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], c[a]); hi[a] = std::max(hi[a], c[a]); }
+    }
+    const double bs = 1.01 * double(thr);
+    int64_t nb[3] = {1, 1, 1};
+    if (n)
+      for (int a = 0; a < 3; ++a) nb[a] = std::max<int64_t>(1, int64_t((hi[a] - lo[a]) / bs) + 1);
+    auto bucket_of = [&](double v, int a) {
+      const int64_t b = int64_t(std::floor((v - lo[a]) / bs));
+      return b < 0 ? int64_t(-1) : (b >= nb[a] ? nb[a] : b);  // -1 / nb[a]: outside
+    };
+    std::vector<std::vector<uint32_t>> bucket(size_t(nb[0] * nb[1] * nb[2]));
+    for (size_t q = 0; q < n; ++q) {  // ascending index inside every bucket
+      const int64_t b0 = bucket_of(p.x[q], 0), b1 = bucket_of(p.y[q], 1), b2 = bucket_of(p.z[q], 2);
+      bucket[size_t((b0 * nb[1] + b1) * nb[2] + b2)].push_back(uint32_t(q));
+    }
+    const size_t nnodes = size_t(fsdf.dims[0]) * fsdf.dims[1] * fsdf.dims[2];
+    std::vector<double> node_out(nnodes * 11, 0.0);
+    std::vector<char> node_on(nnodes, 0);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t t = 0; t < int64_t(nnodes); ++t) {
+      const uint32_t i = uint32_t(t / (int64_t(fsdf.dims[1]) * fsdf.dims[2]));
+      const uint32_t j = uint32_t((t / fsdf.dims[2]) % fsdf.dims[1]);
+      const uint32_t k = uint32_t(t % fsdf.dims[2]);
+      {
+        {
           const R px = fsdf.origin[0] + R(i) * fsdf.cell;
           const R py = fsdf.origin[1] + R(j) * fsdf.cell;
           const R pz = fsdf.origin[2] + R(k) * fsdf.cell;
@@ -413,8 +444,18 @@ template <class R> struct Sim {
           R nx, ny, nz;
           if (!fsdf.gradient(px, py, pz, nx, ny, nz)) continue;
           if (nx == R(0) && ny == R(0) && nz == R(0)) continue;
+          std::vector<uint32_t> cand;
+          const int64_t c0 = bucket_of(px, 0), c1 = bucket_of(py, 1), c2 = bucket_of(pz, 2);
+          for (int64_t a = c0 - 1; a <= c0 + 1; ++a)
+            for (int64_t b = c1 - 1; b <= c1 + 1; ++b)
+              for (int64_t c = c2 - 1; c <= c2 + 1; ++c) {
+                if (a < 0 || b < 0 || c < 0 || a >= nb[0] || b >= nb[1] || c >= nb[2]) continue;
+                const auto& bk = bucket[size_t((a * nb[1] + b) * nb[2] + c)];
+                cand.insert(cand.end(), bk.begin(), bk.end());
+              }
+          std::sort(cand.begin(), cand.end());
           double ps = 0.0, ws = 0.0, g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-          for (size_t q = 0; q < n; ++q) {
+          for (const uint32_t q : cand) {
             const R dx = p.x[q] - px, dy = p.y[q] - py, dz = p.z[q] - pz;
             const R r = std::sqrt(dx * dx + dy * dy + dz * dz);
             if (!(r < thr)) continue;
@@ -433,22 +474,32 @@ template <class R> struct Sim {
           const double pr = ps / ws;
           const double n3[3] = {double(nx), double(ny), double(nz)};
           double f[3];
+          double* o = &node_out[size_t(t) * 11];
           for (int a = 0; a < 3; ++a) {
-            double t = 0.0;
-            for (int b = 0; b < 3; ++b) t += (g[3 * a + b] + g[3 * b + a]) * n3[b];
+            double s = 0.0;
+            for (int b = 0; b < 3; ++b) s += (g[3 * a + b] + g[3 * b + a]) * n3[b];
             const double fp = -pr * n3[a] * double(dA);
-            const double fv = mu * t * double(dA);
-            out[a] += fp;
-            out[3 + a] += fv;
+            const double fv = mu * s * double(dA);
+            o[a] = fp;
+            o[3 + a] = fv;
             f[a] = fp + fv;
           }
           const double rx = px, ry = py, rz = pz;
-          out[6] += ry * f[2] - rz * f[1];
-          out[7] += rz * f[0] - rx * f[2];
-          out[8] += rx * f[1] - ry * f[0];
-          out[9] += 1.0;
-          out[10] += std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+          o[6] = ry * f[2] - rz * f[1];
+          o[7] = rz * f[0] - rx * f[2];
+          o[8] = rx * f[1] - ry * f[0];
+          o[10] = std::sqrt(f[0] * f[0] + f[1] * f[1] + f[2] * f[2]);
+          node_on[size_t(t)] = 1;
         }
+      }
+    }
+    for (size_t t = 0; t < nnodes; ++t) {  // node order (i, j, k), as the serial loop
+      if (!node_on[t]) continue;
+      const double* o = &node_out[t * 11];
+      for (int c = 0; c < 9; ++c) out[c] += o[c];
+      out[9] += 1.0;
+      out[10] += o[10];
+    }
   }
 
   // surrogate feature hook: computed at the end of step()

@@ -60,3 +60,29 @@ def test_surface_forces_need_geometry_and_state():
     e.step_n(1, use_graph=False)
     f = e.surface_forces(mode=1)
     assert np.isfinite(f["net_force"]).all() and np.all(f["viscous_force"] == 0.0)
+
+
+@pytest.mark.timeout(900)
+def test_config3_flow_past_sphere_full_size():
+    # the flow-past-sphere workload at its full size: 256x128x128 lattice
+    # minus the sphere (4.16M particles), sphere SDF 256x128x128
+    sc = scenes.flow_past_sphere()
+    fs = scenes.sphere_sdf(sc["sphere"]["centre"], sc["sphere"]["radius"], sc["sdf"])
+    say(f"case: flow past sphere, full size ({len(sc['x'])} particles), 5 device-dt steps, FP64 forces")
+    e = ubp.HipKernel.from_scene(sc)
+    e.step_n(5, use_graph=True)
+    e.set_force_sdf(fs)
+    got = e.surface_forces(mu=3.05e-2, mode=0)
+    state = e.particles()
+    e.close()
+    say("  engine done; oracle (OpenMP) ...")
+    o = oracle(sc)
+    o.step_auto(5)
+    assert_bitwise(state, o, "flow past sphere, full size, after 5 steps")
+    o.set_force_sdf(fs)
+    want = o.surface_forces(mu=3.05e-2, mode=0)
+    say("  engine", got["net_force"], "oracle", want["net_force"], "samples", got["samples"])
+    assert got["samples"] == want["samples"] > 10000
+    tol = 1e-9 * want["abs_sum"]
+    for k in ("pressure_force", "viscous_force", "net_force"):
+        assert np.all(np.abs(got[k] - want[k]) <= tol), (k, got[k], want[k])
