@@ -194,3 +194,28 @@ def test_profiled_steps_same_bits_and_count():
     assert a.timed_steps == 3
     a.close()
     b.close()
+
+
+@pytest.mark.timeout(900)
+def test_config4_sweep_full_size():
+    # the batched-sweep workload at its full size: 256 dam-break scenes of
+    # 2k-200k particles in one engine, per-scene device dt, graph replay
+    sw = scenes.sweep_scenes()
+    sizes = [len(s["x"]) for s in sw]
+    say(f"case: full sweep, {len(sw)} scenes, {sum(sizes)} particles, 10 device-dt steps, graph")
+    b = ubp.HipBatch(sw)
+    b.step_n(10, use_graph=True)
+    parts = b.scene_particles()
+    dts = b.dts()
+    b.close()
+    for s, sc in enumerate(sw):
+        e = ubp.HipKernel.from_scene(sc)
+        e.step_n(10, use_graph=True)
+        assert_scene_equal(parts[s], e.particles(), f"sweep scene {s} (n={sizes[s]})")
+        assert np.float32(e.dt()) == dts[s], f"sweep scene {s}: dt"
+        e.close()
+    say("  every scene equals its one-scene engine; oracle on the smallest and the largest")
+    for s in (int(np.argmin(sizes)), int(np.argmax(sizes))):
+        o = oracle(sw[s])
+        o.step_auto(10)
+        assert_bitwise(parts[s], o, f"sweep scene {s} (n={sizes[s]}) vs oracle")
