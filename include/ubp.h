@@ -230,6 +230,25 @@ int ubp_get_tuning(const ubp_engine *e, ubp_tuning *out);
 /* Workgroup size per pass (block[UBP_NPASS]); nbr_capacity 0 keeps it. */
 int ubp_set_tuning_ex(ubp_engine *e, const ubp_tuning *t);
 
+/* ---- Runner loop: replaces the stepping of run_simulation_loop
+ * (backend/crates/orchestrator/src/runner.rs:157-277) with the dt kept on
+ * the device.  dt_safe is the CFL dt of the current state
+ * (compute_timestep, kernel/src/sph.rs:591-628).  Outside a cooldown each
+ * step is tried at alpha * dt_safe from a device checkpoint (separate from
+ * ubp_save_checkpoint's); when max |rho - rho0| / rho0 then exceeds
+ * density_limit it is rolled back and taken at dt_safe, and the next
+ * `cooldown` steps run at dt_safe.  The reference uses alpha 1.5, limit
+ * 0.03, cooldown 10.  Stops after max_steps steps (0: none), then once the
+ * simulated time reaches max_time (<= 0: none).  One-scene engines.
+ * Synchronous. */
+typedef struct {
+  uint64_t steps, accepted, rejected;
+  double sim_time;
+} ubp_run_stats;
+
+int ubp_run_optimistic(ubp_engine *e, uint64_t max_steps, double max_time, float alpha, float density_limit,
+                       uint32_t cooldown, ubp_run_stats *stats);
+
 /* ---- Viewer frame: replaces build_frame (backend/crates/server/src/ws.rs:
  * 232-294) over a full particles() readback.  Writes the Frame message
  * (tag 0x02, frame number u64, particle count u32, sim time f64, then per

@@ -54,7 +54,8 @@ enum : uint32_t {
   ORC_EOS_EXACT = 2u,      // (rho/rho0)^gamma - 1 evaluated in FP64
   ORC_DIFF_RHO_FLOOR = 4u, // delta-SPH divides by max(rho_j, floor) (wgpu variant)
   ORC_MORTON = 8u,         // debug keys are Morton codes (else linear hash)
-  ORC_ENGINE_CHECKPOINT = 16u, // restore also restores the neighbour grid
+  ORC_ENGINE_CHECKPOINT = 16u, // restore also restores the neighbour grid and whether the
+                               // bootstrap force pass has run (a complete state)
 };
 
 template <class R> static inline R pi_c() { return R(3.14159265358979323846); }
@@ -382,6 +383,7 @@ template <class R> struct Sim {
   double e0 = 0, m0 = 0;
   bool needs_init = true;
   bool has_ckpt = false;
+  bool ckpt_needs_init = true;
   State<R> ckpt;
   Grid<R> ckpt_grid;
   std::vector<uint32_t> last_nbr_count;
@@ -1178,10 +1180,16 @@ void orc_grid_dims(void* h, uint32_t dims[3]) {
   else { for (int a = 0; a < 3; ++a) dims[a] = hd->f->grid.dims[a]; }
 }
 
-// lib.rs:383-395; ORC_ENGINE_CHECKPOINT additionally snapshots the grid
+// lib.rs:383-395 (the particle arrays only); ORC_ENGINE_CHECKPOINT
+// additionally snapshots the grid and the bootstrap flag
 int orc_save_checkpoint(void* h) {
   Handle* hd = static_cast<Handle*>(h);
-  auto run = [&](auto* s) { s->ckpt = s->p; s->ckpt_grid = s->grid; s->has_ckpt = true; };
+  auto run = [&](auto* s) {
+    s->ckpt = s->p;
+    s->ckpt_grid = s->grid;
+    s->ckpt_needs_init = s->needs_init;
+    s->has_ckpt = true;
+  };
   if (hd->d) run(hd->d); else run(hd->f);
   return 1;
 }
@@ -1191,7 +1199,10 @@ int orc_restore_checkpoint(void* h) {
   auto run = [&](auto* s) {
     if (!s->has_ckpt) return;
     s->p = s->ckpt;
-    if (s->prm.flags & ORC_ENGINE_CHECKPOINT) s->grid = s->ckpt_grid;
+    if (s->prm.flags & ORC_ENGINE_CHECKPOINT) {
+      s->grid = s->ckpt_grid;
+      s->needs_init = s->ckpt_needs_init;
+    }
     s->has_ckpt = false;
     ok = 1;
   };

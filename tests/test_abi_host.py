@@ -149,6 +149,11 @@ def test_struct_layouts_match_header(tmp_path):
         ' printf("%zu %zu %zu\\n", sizeof(ubp_tuning), offsetof(ubp_tuning, nbr_capacity),'
         " offsetof(ubp_tuning, step_ms));\n"
         ' printf("%zu %zu\\n", sizeof(ubp_particles_in), offsetof(ubp_particles_in, fluid_type));\n'
+        ' printf("%zu %zu %zu\\n", sizeof(ubp_domain), offsetof(ubp_domain, fluid_type),'
+        " offsetof(ubp_domain, walls));\n"
+        ' printf("%zu %zu\\n", sizeof(ubp_run_stats), offsetof(ubp_run_stats, sim_time));\n'
+        ' printf("%zu %zu\\n", sizeof(ubp_boundary_out), offsetof(ubp_boundary_out, nz));\n'
+        ' printf("%zu %zu\\n", sizeof(ubp_surface_force), offsetof(ubp_surface_force, samples));\n'
         " return 0; }\n")
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
@@ -158,6 +163,11 @@ def test_struct_layouts_match_header(tmp_path):
     assert got[0] == [C.sizeof(S), S.alpha.offset, S.sdf.offset, S.sdf_cell.offset, S.sdf_dims.offset]
     assert got[1] == [C.sizeof(T), T.nbr_capacity.offset, T.step_ms.offset]
     assert got[2] == [C.sizeof(P), P.fluid_type.offset]
+    D, R, B, F = ubp._Domain, ubp._RunStats, ubp._BoundaryOut, ubp._SurfaceForce
+    assert got[3] == [C.sizeof(D), D.fluid_type.offset, D.walls.offset]
+    assert got[4] == [C.sizeof(R), R.sim_time.offset]
+    assert got[5] == [C.sizeof(B), B.nz.offset]
+    assert got[6] == [C.sizeof(F), F.samples.offset]
 
 
 @pytest.mark.skipif(ubp.gpu_available(), reason="CPU-only check")

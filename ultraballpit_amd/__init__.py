@@ -125,6 +125,10 @@ class _SurfaceForce(C.Structure):
                 ("net_force", C.c_double * 3), ("net_moment", C.c_double * 3), ("samples", C.c_uint64)]
 
 
+class _RunStats(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("accepted", C.c_uint64), ("rejected", C.c_uint64), ("sim_time", C.c_double)]
+
+
 class _Domain(C.Structure):
     _fields_ = [("domain_min", C.c_float * 3), ("domain_max", C.c_float * 3), ("particle_spacing", C.c_float),
                 ("fluid_type", C.c_int32), ("initial_temperature", C.c_float), ("walls", C.c_uint32)]
@@ -157,7 +161,7 @@ EXPORTS = ("ubp_create", "ubp_destroy", "ubp_last_error", "ubp_gpu_available",
            "ubp_surrogate_step", "ubp_surrogate_features", "ubp_trainer_create", "ubp_trainer_destroy",
            "ubp_trainer_collect", "ubp_trainer_finish", "ubp_trainer_rows", "ubp_trainer_init_model", "ubp_trainer_fit",
            "ubp_trainer_get_model", "ubp_set_force_sdf", "ubp_surface_forces", "ubp_build_frame",
-           "ubp_setup_domain", "ubp_sdf_analytic")
+           "ubp_setup_domain", "ubp_sdf_analytic", "ubp_run_optimistic")
 
 #: kernel order of ubp_step_n_profiled
 KERNELS = ("k_kick", "k_xsph", "k_drift", "k_scan", "k_scatter", "k_rank_gather", "k_neigh", "k_expand",
@@ -240,6 +244,8 @@ def lib():
     L.ubp_build_frame.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64, C.c_double, C.POINTER(C.c_uint8),
                                   C.c_size_t, C.POINTER(C.c_size_t)]
     L.ubp_surface_forces.argtypes = [C.c_void_p, C.c_float, C.c_int, C.POINTER(_SurfaceForce)]
+    L.ubp_run_optimistic.argtypes = [C.c_void_p, C.c_uint64, C.c_double, C.c_float, C.c_float, C.c_uint32,
+                                     C.POINTER(_RunStats)]
     L.ubp_setup_domain.argtypes = [C.POINTER(_Domain), fp, C.c_float, u32p, fp, C.POINTER(_ParticlesOut),
                                    C.c_size_t, C.POINTER(_BoundaryOut), C.c_size_t, C.POINTER(C.c_size_t),
                                    C.POINTER(C.c_size_t)]
@@ -522,6 +528,16 @@ class HipKernel:
 
     def particle_count(self) -> int:
         return int(lib().ubp_particle_count(self._h))
+
+    def run_optimistic(self, max_steps: int = 0, max_time: float = 0.0, alpha: float = 1.5,
+                       density_limit: float = 0.03, cooldown: int = 10) -> dict:
+        """The orchestrator's optimistic stepping loop (runner.rs:157-277) with
+        the dt on the device; returns steps, accepted, rejected, sim_time."""
+        s = _RunStats()
+        _check(lib().ubp_run_optimistic(self._h, int(max_steps), float(max_time), float(alpha),
+                                        float(density_limit), int(cooldown), C.byref(s)), "run_optimistic")
+        return {"steps": int(s.steps), "accepted": int(s.accepted), "rejected": int(s.rejected),
+                "sim_time": float(s.sim_time)}
 
     def save_checkpoint(self) -> bool:
         return _check(lib().ubp_save_checkpoint(self._h), "save_checkpoint") == 1

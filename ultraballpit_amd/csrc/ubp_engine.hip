@@ -202,7 +202,7 @@ struct Engine {
   uint32_t *bl_start = nullptr, *bl = nullptr, *bc_start = nullptr, *bc = nullptr;
 
   // checkpoint (reference save/restore_checkpoint) and autotuner snapshot
-  Snapshot ck, tune_ck;
+  Snapshot ck, tune_ck, run_ck;
 
   // learned surrogate: last-correction state (gathered with the particles),
   // feature record and features, judge buffers, the network
@@ -269,6 +269,7 @@ struct Engine {
     dfree(d_sp); dfree(sid[0]); dfree(sid[1]); dfree(d_soff); dfree(sacc);
     ck.release();
     tune_ck.release();
+    run_ck.release();
     sur_ck.release();
     dfree(aux[0]); dfree(aux[1]); dfree(sr); dfree(feat); dfree(x0); dfree(jpart); dfree(dts); dfree(inv);
     dfree(accx); dfree(xref);
@@ -1365,6 +1366,64 @@ struct Engine {
     for (size_t s = 0; s < n; ++s) std::memcpy(out + size_t(ho[s]) * 32, hf.data() + s * 32, 128);
   }
 
+  // ---- runner loop (orchestrator/src/runner.rs:185-277) ----
+  // The stepping of run_simulation_loop with the dt on the device.  dt_safe
+  // is the CFL dt of the current state (compute_timestep, sph.rs:591-628),
+  // which every device-dt step leaves in device memory.  Outside a cooldown
+  // the step is tried at alpha dt_safe from a device checkpoint of its own;
+  // if max |rho - rho0| / rho0 then exceeds `limit` it is rolled back and
+  // taken at dt_safe, and the next `cooldown` steps run at dt_safe.  Stops
+  // after max_steps steps (0: no limit), then once the simulated time
+  // reaches max_time (<= 0: no limit).  Per step the dt (4 bytes) and the
+  // per-block density maxima cross PCIe.
+  void run_optimistic(uint64_t max_steps, double max_time, float alpha, float limit, uint32_t cooldown,
+                      ubp_run_stats* R) {
+    if (multi) throw Error(UBP_EINVAL, "the runner loop drives one-scene engines");
+    if (max_steps == 0 && !(max_time > 0.0)) throw Error(UBP_EINVAL, "give max_steps or max_time");
+    if (!(alpha > 0.0f)) throw Error(UBP_EINVAL, "alpha must be positive");
+    std::memset(R, 0, sizeof(*R));
+    uint32_t cool = 0;
+    auto one_step = [&]() {
+      if (needs_init) bootstrap();
+      step_body(1);  // also leaves the CFL dt of the new state in dtp
+      dt_valid = true;
+    };
+    while (true) {
+      stepped = true;
+      ensure_dt();
+      float dt_safe = 0.0f;
+      HIPCHK(hipMemcpyAsync(&dt_safe, dtp, 4, hipMemcpyDeviceToHost, st));
+      sync();
+      float dt_used = dt_safe;
+      if (cool > 0 || n == 0) {
+        if (cool > 0) --cool;
+        if (n) one_step();
+      } else {
+        save_snap(run_ck);
+        const float dt_try = dt_safe * alpha;
+        set_dt(dt_try);
+        one_step();
+        ubp_error_metrics_t m;
+        metrics(&m);
+        if (m.max_density_variation > limit) {
+          restore_snap(run_ck);
+          set_dt(dt_safe);
+          one_step();
+          ++R->rejected;
+          cool = cooldown;
+        } else {
+          dt_used = dt_try;
+          ++R->accepted;
+        }
+      }
+      R->sim_time += double(dt_used);
+      ++R->steps;
+      if (max_steps && R->steps >= max_steps) break;
+      if (max_time > 0.0 && R->sim_time >= max_time) break;
+    }
+    sync();
+  }
+
   // ---- viewer frame (backend/crates/server/src/ws.rs:232-294) ----
   // Frame message: tag 0x02, frame number u64, particle count u32, sim time
   // f64, then 20 bytes per subsampled particle (original index k * stride,
@@ -2092,6 +2151,16 @@ int ubp_sdf_analytic(int kind, const double* params, const double origin[3], dou
   LAUNCHCHK("k_sdf_analytic");
   HIPCHK(hipMemcpyAsync(out, dout.p, size_t(n) * 4, hipMemcpyDeviceToHost, S.s));
   HIPCHK(hipStreamSynchronize(S.s));
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_run_optimistic(ubp_engine* e, uint64_t max_steps, double max_time, float alpha, float density_limit,
+                       uint32_t cooldown, ubp_run_stats* stats) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  if (!stats) throw Error(UBP_EINVAL, "null stats");
+  e->eng.run_optimistic(max_steps, max_time, alpha, density_limit, cooldown, stats);
   return UBP_OK;
   UBP_API_END
 }
