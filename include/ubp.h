@@ -230,6 +230,12 @@ int ubp_get_tuning(const ubp_engine *e, ubp_tuning *out);
 /* Workgroup size per pass (block[UBP_NPASS]); nbr_capacity 0 keeps it. */
 int ubp_set_tuning_ex(ubp_engine *e, const ubp_tuning *t);
 
+/* Device check of the server's instability test (backend/crates/server/
+ * src/runner.rs:190-205) without a particles() readback: *finite = 1 when
+ * every x, y, z, vx, vy, vz, density and pressure is finite, else 0.
+ * Synchronous. */
+int ubp_state_finite(ubp_engine *e, int *finite);
+
 /* ---- Runner loop: replaces the stepping of run_simulation_loop
  * (backend/crates/orchestrator/src/runner.rs:157-277) with the dt kept on
  * the device.  dt_safe is the CFL dt of the current state

@@ -161,7 +161,8 @@ EXPORTS = ("ubp_create", "ubp_destroy", "ubp_last_error", "ubp_gpu_available",
            "ubp_surrogate_step", "ubp_surrogate_features", "ubp_trainer_create", "ubp_trainer_destroy",
            "ubp_trainer_collect", "ubp_trainer_finish", "ubp_trainer_rows", "ubp_trainer_init_model", "ubp_trainer_fit",
            "ubp_trainer_get_model", "ubp_set_force_sdf", "ubp_surface_forces", "ubp_build_frame",
-           "ubp_setup_domain", "ubp_sdf_analytic", "ubp_run_optimistic")
+           "ubp_setup_domain", "ubp_sdf_analytic", "ubp_run_optimistic",
+           "ubp_state_finite")
 
 #: kernel order of ubp_step_n_profiled
 KERNELS = ("k_kick", "k_xsph", "k_drift", "k_scan", "k_scatter", "k_rank_gather", "k_neigh", "k_expand",
@@ -244,6 +245,7 @@ def lib():
     L.ubp_build_frame.argtypes = [C.c_void_p, C.c_uint32, C.c_uint64, C.c_double, C.POINTER(C.c_uint8),
                                   C.c_size_t, C.POINTER(C.c_size_t)]
     L.ubp_surface_forces.argtypes = [C.c_void_p, C.c_float, C.c_int, C.POINTER(_SurfaceForce)]
+    L.ubp_state_finite.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
     L.ubp_run_optimistic.argtypes = [C.c_void_p, C.c_uint64, C.c_double, C.c_float, C.c_float, C.c_uint32,
                                      C.POINTER(_RunStats)]
     L.ubp_setup_domain.argtypes = [C.POINTER(_Domain), fp, C.c_float, u32p, fp, C.POINTER(_ParticlesOut),
@@ -528,6 +530,12 @@ class HipKernel:
 
     def particle_count(self) -> int:
         return int(lib().ubp_particle_count(self._h))
+
+    def state_finite(self) -> bool:
+        """Every x, y, z, vx, vy, vz, density and pressure is finite (device check)."""
+        f = C.c_int(0)
+        _check(lib().ubp_state_finite(self._h, C.byref(f)), "state_finite")
+        return bool(f.value)
 
     def run_optimistic(self, max_steps: int = 0, max_time: float = 0.0, alpha: float = 1.5,
                        density_limit: float = 0.03, cooldown: int = 10) -> dict:

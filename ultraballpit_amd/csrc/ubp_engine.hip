@@ -203,6 +203,7 @@ struct Engine {
 
   // checkpoint (reference save/restore_checkpoint) and autotuner snapshot
   Snapshot ck, tune_ck, run_ck;
+  uint32_t* finite_flag = nullptr;
 
   // learned surrogate: last-correction state (gathered with the particles),
   // feature record and features, judge buffers, the network
@@ -270,6 +271,7 @@ struct Engine {
     ck.release();
     tune_ck.release();
     run_ck.release();
+    dfree(finite_flag);
     sur_ck.release();
     dfree(aux[0]); dfree(aux[1]); dfree(sr); dfree(feat); dfree(x0); dfree(jpart); dfree(dts); dfree(inv);
     dfree(accx); dfree(xref);
@@ -1366,6 +1368,23 @@ struct Engine {
     for (size_t s = 0; s < n; ++s) std::memcpy(out + size_t(ho[s]) * 32, hf.data() + s * 32, 128);
   }
 
+  // true when every x, y, z, vx, vy, vz, density and pressure is finite
+  bool state_finite() {
+    if (n == 0) return true;
+    if (!finite_flag) dalloc(&finite_flag, 1);
+    const uint32_t z = 0, one = 1;
+    HIPCHK(hipMemcpyAsync(finite_flag, &z, 4, hipMemcpyHostToDevice, st));
+    const float4* p = needs_init ? pos_t : pos;
+    const float4* v = needs_init ? vel_t : vel;
+    hipLaunchKernelGGL(k_finite, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, st, uint32_t(n), p, v,
+                       (const float*)rho_n, (const float*)prs, finite_flag);
+    LAUNCHCHK("k_finite");
+    uint32_t bad = one;
+    HIPCHK(hipMemcpyAsync(&bad, finite_flag, 4, hipMemcpyDeviceToHost, st));
+    sync();
+    return bad == 0u;
+  }
+
   // ---- runner loop (orchestrator/src/runner.rs:185-277) ----
   // The stepping of run_simulation_loop with the dt on the device.  dt_safe
   // is the CFL dt of the current state (compute_timestep, sph.rs:591-628),
@@ -2151,6 +2170,15 @@ int ubp_sdf_analytic(int kind, const double* params, const double origin[3], dou
   LAUNCHCHK("k_sdf_analytic");
   HIPCHK(hipMemcpyAsync(out, dout.p, size_t(n) * 4, hipMemcpyDeviceToHost, S.s));
   HIPCHK(hipStreamSynchronize(S.s));
+  return UBP_OK;
+  UBP_API_END
+}
+
+int ubp_state_finite(ubp_engine* e, int* finite) {
+  UBP_API_BEGIN
+  UBP_HANDLE_CHECK(e);
+  if (!finite) throw Error(UBP_EINVAL, "null argument");
+  *finite = e->eng.state_finite() ? 1 : 0;
   return UBP_OK;
   UBP_API_END
 }

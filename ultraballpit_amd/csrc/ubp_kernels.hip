@@ -1492,6 +1492,21 @@ __global__ UBP_LB void k_frame(DevParams P, uint32_t count, uint32_t stride, con
 
 // ---- diagnostics ----
 // Plain streaming copy (HBM counter calibration).
+// Any non-finite x, y, z, vx, vy, vz, density or pressure sets *bad
+// (server/src/runner.rs:190-205).
+__global__ UBP_LB void k_finite(uint32_t n, const float4* __restrict__ pos, const float4* __restrict__ vel,
+                                const float* __restrict__ rho, const float* __restrict__ prs,
+                                uint32_t* __restrict__ bad) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool ok = true;
+  if (i < n) {
+    const float4 p = pos[i], v = vel[i];
+    ok = isfinite(p.x) && isfinite(p.y) && isfinite(p.z) && isfinite(v.x) && isfinite(v.y) && isfinite(v.z) &&
+         isfinite(rho[i]) && isfinite(prs[i]);
+  }
+  if (__ballot(!ok) != 0ull && (threadIdx.x & 63u) == 0u) atomicOr(bad, 1u);
+}
+
 __global__ UBP_LB void k_calib_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n4) {
   size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   size_t stride = size_t(gridDim.x) * blockDim.x;
