@@ -482,7 +482,6 @@ __global__ __launch_bounds__(64) void k_expand(DevParams P0, Scenes S, const flo
   }
 }
 
-
 // Density summation + delta-SPH (old density) + SDF wall volume + Tait EOS.
 // sph.rs:112-160 (density), sph.rs:169-216 (diffusion), lib.rs:232-237,
 // eos.rs:39-47.
