@@ -50,6 +50,9 @@ def main():
     ap.add_argument("--out", default=os.path.join(HERE, "models", "dam_break_surrogate.npz"))
     ap.add_argument("--particles", default="262144,262144,262144,262144,1048576,1048576",
                     help="particle count of each rollout")
+    ap.add_argument("--rollouts", type=int, default=0,
+                    help="instead of --particles: this many rollouts of --rollout-particles each")
+    ap.add_argument("--rollout-particles", type=int, default=262144)
     ap.add_argument("--warm", type=int, default=10, help="settling cycles before collection")
     ap.add_argument("--cycles", type=int, default=10)
     ap.add_argument("--frac", type=float, default=0.125)
@@ -62,7 +65,8 @@ def main():
     if not ubp.gpu_available():
         sys.exit("no gfx950 device")
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
-    counts = [int(x) for x in args.particles.split(",")]
+    counts = ([args.rollout_particles] * args.rollouts if args.rollouts > 0
+              else [int(x) for x in args.particles.split(",")])
     sc = rollout_scenes(counts)
     t0 = time.perf_counter()
     model, losses, info = surrogate.train(sc, cycles=args.cycles, steps=args.steps, batch=args.batch, lr=args.lr,
