@@ -368,6 +368,19 @@ __device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, fl
   }
 }
 
+// Positions through a raw buffer resource: record (lane_off + uni_off) / 16,
+// with the per-lane byte offset in a VGPR and the wave-uniform one in an
+// SGPR, so a walk over a cell costs no per-lane address arithmetic.  The
+// resource covers the n records (n <= 2^28: byte offsets fit 32 bits).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pos_rsrc(const float4* pos, uint32_t n) {
+  const uint32_t bytes = n >= (1u << 28) ? 0xffffffffu : n * 16u;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<float4*>(pos), 0, int(bytes), 0x00020000);
+}
+__device__ __forceinline__ float3 ld_pos(__amdgpu_buffer_rsrc_t r, uint32_t lane_off, uint32_t uni_off) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, lane_off, uni_off, 0);
+  return make_float3(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]));
+}
+
 template <bool M>
 __global__ UBP_LB void k_neigh(DevParams P0, Scenes S, const float4* __restrict__ pos,
                                const uint32_t* __restrict__ start, uint32_t* __restrict__ nmask,
@@ -383,28 +396,41 @@ __global__ UBP_LB void k_neigh(DevParams P0, Scenes S, const float4* __restrict_
   nbhd(P, cx, cy, cz, N);
   uint32_t c = 0, ovf = 0;
   uint32_t* m = nmask + i;
+  const __amdgpu_buffer_rsrc_t prs = pos_rsrc(pos, P0.n);
 #pragma unroll 1
   for (int k = 0; k < 27; ++k) {
     const int dz = k / 9, dy = (k / 3) % 3, dx = k % 3;
     const uint32_t sx = sel3(N.sx, dx), sy = sel3(N.sy, dy), sz = sel3(N.sz, dz);
-    uint32_t mask = 0u;
+    uint32_t b = 0u, cnt = 0u;
     if (sx != 0xffffffffu && sy != 0xffffffffu && sz != 0xffffffffu) {
       const uint32_t code = P.code_base + (sx | sy | sz);
-      const uint32_t b = start[code], e = start[code + 1u];
-      if (e - b > 32u) ovf = UBP_NN_OVF;
-      uint32_t bit = 1u;
-      for (uint32_t s = b; s < e; ++s, bit <<= 1) {
-        if (s == i) continue;
-        float4 q = pos[s];
-        float ddx = pi.x - q.x;
-        float ddy = pi.y - q.y;
-        float ddz = pi.z - q.z;
-        float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-        if (d2 <= P.rsq) {
-          ++c;
-          mask |= bit;  // bit is 0 beyond the 32nd slot of the cell
-        }
-      }
+      b = start[code];
+      cnt = start[code + 1u] - b;
+    }
+    if (cnt > 32u) ovf = UBP_NN_OVF;
+    const uint32_t lim = cnt < 32u ? cnt : 32u;
+    const uint32_t boff = b * 16u;  // byte offset of the cell's first record
+    uint32_t mask = 0u;
+    // t is the same in every active lane: the record offset t * 16 is a
+    // scalar operand; i's own slot is tested too (distance 0) and cleared
+    // below, and the count is the mask's popcount
+    for (uint32_t t = 0; t < lim; ++t) {
+      const float3 q = ld_pos(prs, boff, t * 16u);
+      const float ddx = pi.x - q.x;
+      const float ddy = pi.y - q.y;
+      const float ddz = pi.z - q.z;
+      const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+      mask = (d2 <= P.rsq) ? (mask | (1u << t)) : mask;
+    }
+    if (k == 13 && i - b < 32u) mask &= ~(1u << (i - b));  // i itself (distance 0) is no neighbour
+    c += uint32_t(__popc(mask));
+    for (uint32_t t = 32u; t < cnt; ++t) {  // slots past the 32nd are counted only
+      const float3 q = ld_pos(prs, boff, t * 16u);
+      const float ddx = pi.x - q.x;
+      const float ddy = pi.y - q.y;
+      const float ddz = pi.z - q.z;
+      const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+      if (d2 <= P.rsq && b + t != i) ++c;
     }
     m[size_t(k) * P.n] = mask;
   }
