@@ -69,6 +69,48 @@ def test_bins_keys_perm_counts_bitwise():
     assert_bitwise(e.particles(), o, "dam_break_small after 20 steps")
 
 
+def _dam_break_with_dense_cluster():
+    """dam_break_small plus an 8^3 lattice at half the spacing: its cells
+    hold far more than 32 particles (neighbour masks overflow there and those
+    particles re-walk their cells), next to ordinary cells."""
+    sc = scenes.dam_break_small(lat=(12, 12, 24))
+    dx = sc["dx"]
+    cl = scenes._particles(scenes.lattice(8, 8, 8, 0.5 * dx, origin=(0.7, 0.05, 0.05)), 0.5 * dx)
+    for k, v in cl.items():
+        sc[k] = np.concatenate([sc[k], v])
+    return sc
+
+
+def test_dense_cells_past_32_slots_bitwise():
+    say("case: dam_break_small + 512-particle dense cluster (cells over 32 particles), 12 steps")
+    sc = _dam_break_with_dense_cluster()
+    e = engine(sc)
+    o = oracle(sc)
+    e.step_n(12, use_graph=False)
+    o.step_auto(12)
+    ek, ep, ec = e.debug_bins()
+    ok, op, oc = o.bins(morton=True)
+    assert np.bincount(ek).max() > 32, "the scene must have cells with more than 32 particles"
+    assert np.array_equal(ek, ok), "cell keys differ"
+    assert np.array_equal(ep, op), "sorted permutation differs"
+    assert np.array_equal(ec, oc), "neighbour counts differ"
+    assert ec.max() > 64, "cluster particles have more neighbours than one 32-slot cell mask holds"
+    assert_bitwise(e.particles(), o, "dense cluster after 12 steps")
+
+
+def test_list_capacity_below_neighbour_counts_bitwise():
+    say("case: dam_break_small with list capacity 64 (interior particles re-walk their cells), 12 graph steps")
+    sc = scenes.dam_break_small(lat=(12, 12, 24))
+    e = engine(sc)
+    o = oracle(sc)
+    e.set_tuning(0, 64)
+    e.step_n(12, use_graph=True)
+    o.step_auto(12)
+    _, _, ec = e.debug_bins()
+    assert (ec > 64).any() and (ec <= 64).any(), "both list walks and cell re-walks must occur"
+    assert_bitwise(e.particles(), o, "capacity 64 after 12 steps")
+
+
 def test_config1_water_box_100_steps():
     say("case: water box 20^3 = 8000 particles, 100 KDK steps, CFL 0.25 device dt")
     sc = scenes.water_box()
