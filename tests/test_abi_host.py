@@ -175,3 +175,58 @@ def test_batch_create_fails_cleanly_without_device():
     sw = scenes.sweep_scenes(n_scenes=2, n_min=100, n_max=300)
     with pytest.raises(ubp.GpuInitError, match="device"):
         ubp.HipBatch(sw)
+
+
+def _handle_entry_points():
+    """(name, parameter count, return type) of every header function whose
+    first parameter is an engine or trainer handle."""
+    txt = open(os.path.join(ROOT, "include", "ubp.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    out = []
+    for ret, name, params in re.findall(r"\b(int|void|uint32_t)\s+(ubp_[a-z0-9_]+)\s*\(([^;{]*?)\)\s*;", txt, flags=re.S):
+        first = params.split(",")[0]
+        if re.search(r"\bubp_(engine|trainer)\s*\*", first):
+            out.append((name, params.count(",") + 1, ret))
+    return out
+
+
+def test_null_handles_are_rejected_without_a_device():
+    """Every entry point that takes an engine or trainer handle returns
+    UBP_EINVAL with a message for a null handle (void ones return), touching
+    no device."""
+    lib = C.CDLL(ubp.LIB_PATH)
+    lib.ubp_last_error.restype = C.c_char_p
+    eps = _handle_entry_points()
+    assert len(eps) >= 40, eps
+    for name, nargs, ret in eps:
+        fn = getattr(lib, name)
+        fn.argtypes = [C.c_void_p] * nargs
+        fn.restype = {"int": C.c_int, "uint32_t": C.c_uint32, "void": None}[ret]
+        r = fn(*([None] * nargs))
+        if ret == "int":
+            assert r == -1, f"{name}: {r}"
+            assert lib.ubp_last_error(), name
+        elif ret == "uint32_t":
+            assert r == 0, name
+
+
+def test_null_arguments_are_rejected_without_a_device():
+    """Entry points without a handle reject null arguments with UBP_EINVAL and
+    a message; the constructors return NULL with a message."""
+    lib = C.CDLL(ubp.LIB_PATH)
+    lib.ubp_last_error.restype = C.c_char_p
+    txt = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "ubp.h")).read(), flags=re.S)
+    for name in ("ubp_setup_domain", "ubp_sdf_analytic", "ubp_mlp_forward", "ubp_mlp_loss_grad",
+                 "ubp_mfma_selftest", "ubp_calibration_copy"):
+        nargs = re.search(r"\b" + name + r"\s*\(([^;]*?)\)\s*;", txt, flags=re.S).group(1).count(",") + 1
+        fn = getattr(lib, name)
+        fn.argtypes = [C.c_void_p] * nargs
+        fn.restype = C.c_int
+        assert fn(*([None] * nargs)) == -1, name
+        assert lib.ubp_last_error(), name
+    for name, nargs in (("ubp_create", 9), ("ubp_create_batch", 2), ("ubp_trainer_create", 2)):
+        fn = getattr(lib, name)
+        fn.argtypes = [C.c_void_p] * nargs
+        fn.restype = C.c_void_p
+        assert fn(*([None] * nargs)) is None, name
+        assert lib.ubp_last_error(), name
