@@ -260,6 +260,35 @@ def sphere_bench(steps: int):
             "samples": f["samples"]}
 
 
+def slab_bench(steps: int, warmup: int):
+    """One GPU's share of the flow past a sphere: slab 3 of 8 along x (the
+    one holding the sphere's downstream half) with the 3 x 2h halo on each
+    side that a once-per-step exchange needs; graph-replayed device-dt steps.
+    Reports the owned particles' rate (the rate one GPU contributes to an
+    8-GPU run, exchange excluded) and the halo's share of the work."""
+    sc = scenes.flow_past_sphere_slab(slab=2, nslabs=8)
+    s = sc["slab"]
+    n = len(sc["x"])
+    e = ubp.HipKernel.from_scene(sc)
+    e.step_n(warmup, use_graph=True)
+    e.sync()
+    t0 = time.perf_counter()
+    e.step_n(steps, use_graph=True)
+    e.sync()
+    t1 = time.perf_counter()
+    finite = e.state_finite()
+    e.close()
+    ms = (t1 - t0) * 1e3 / steps
+    return {"workload": f"flow past a sphere, slab {s['index'] + 1} of {s['of']} along x "
+                        f"(x in [{s['x'][0]:.3f}, {s['x'][1]:.3f}] m) plus a {1e3 * s['halo']:.2f} mm (6 h) halo "
+                        "on each side; slab faces are walls in this one-GPU stand-in",
+            "owned": s["owned"], "halo": s["halo_particles"], "particles": n, "steps": steps,
+            "ms_per_step": ms, "value": s["owned"] * steps / (t1 - t0),
+            "unit": "owned particle-steps/s on one GPU (exchange excluded)",
+            "all_particle_steps_per_s": n * steps / (t1 - t0), "halo_fraction": s["halo_particles"] / n,
+            "state_finite": finite}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -276,6 +305,7 @@ def main():
     ap.add_argument("--autotune-steps", type=int, default=3)
     ap.add_argument("--surrogate-cycles", type=int, default=8, help="0 skips the surrogate leg")
     ap.add_argument("--sphere-steps", type=int, default=40, help="0 skips the flow-past-sphere leg")
+    ap.add_argument("--slab-steps", type=int, default=100, help="0 skips the one-slab (1 of 8) leg")
     ap.add_argument("--surrogate-warm", type=int, default=660)
     ap.add_argument("--surrogate-model", default=os.path.join(ROOT, "ultraballpit_amd", "models",
                                                               "dam_break_surrogate.npz"))
@@ -368,6 +398,9 @@ def main():
     if args.sphere_steps > 0:
         log(f"[bench] flow past sphere, {args.sphere_steps} steps with force extraction")
         out["sphere"] = sphere_bench(args.sphere_steps)
+    if args.slab_steps > 0:
+        log(f"[bench] flow past sphere, slab 3 of 8 with halo, {args.slab_steps} steps")
+        out["slab"] = slab_bench(args.slab_steps, min(args.warmup, 10))
     if args.surrogate_cycles > 0:
         log(f"[bench] surrogate leg, {args.surrogate_cycles} cycles")
         out["surrogate"] = surrogate_bench(args.surrogate_cycles, args.surrogate_warm, args.surrogate_model)

@@ -230,3 +230,26 @@ def test_null_arguments_are_rejected_without_a_device():
         fn.restype = C.c_void_p
         assert fn(*([None] * nargs)) is None, name
         assert lib.ubp_last_error(), name
+
+
+def test_slab_scenes_partition_the_channel():
+    """The eight one-GPU slabs of the flow past a sphere own every particle of
+    the full scene exactly once, and each slab's halo lies within 6 h of it."""
+    kw = dict(lat=(64, 32, 32), sdf_nodes=(64, 32, 32))
+    full = scenes.flow_past_sphere(**kw)
+    h = float(full["h"])
+    owned_total = 0
+    seen = []
+    for k in range(8):
+        sc = scenes.flow_past_sphere_slab(slab=k, nslabs=8, **kw)
+        s = sc["slab"]
+        x = sc["x"].astype(np.float64)
+        assert s["owned"] + s["halo_particles"] == len(x)
+        assert np.all((x >= s["x"][0] - 6 * h - 1e-9) & (x < s["x"][1] + 6 * h + 1e-9))
+        assert np.all((x[sc["owned"]] >= s["x"][0]) & (x[sc["owned"]] < s["x"][1]))
+        owned_total += s["owned"]
+        seen.append(np.stack([sc["x"][sc["owned"]], sc["y"][sc["owned"]], sc["z"][sc["owned"]]], 1))
+    assert owned_total == len(full["x"])
+    got = np.concatenate(seen)
+    want = np.stack([full["x"], full["y"], full["z"]], 1)
+    assert np.array_equal(got[np.lexsort(got.T)], want[np.lexsort(want.T)])

@@ -266,6 +266,37 @@ def flow_past_sphere(lat=(256, 128, 128), channel=(0.4, 0.2, 0.2), centre=(0.1, 
     return sc
 
 
+def flow_past_sphere_slab(slab=2, nslabs=8, halo_h=6.0, **kw):
+    """One GPU's share of the flow past a sphere cut into ``nslabs`` slabs
+    along x (the channel's longest axis): the particles of the full scene
+    (same lattice, same seeded noise) with x in the slab widened by
+    ``halo_h`` * h on each side.  6 h = 3 x 2h is the halo a step needs when
+    the halo is exchanged once per step (XSPH -> drift -> density -> forces).
+    The slab domain and its box-and-sphere SDF end at the widened slab, so
+    its faces act as walls here; in a sharded run they are exchange faces.
+    ``owned`` marks the slab's own particles."""
+    full = flow_past_sphere(**kw)
+    channel = full["domain_max"]
+    w = channel[0] / nslabs
+    x0, x1 = slab * w, (slab + 1) * w
+    halo = halo_h * float(full["h"])
+    lo, hi = max(0.0, x0 - halo), min(channel[0], x1 + halo)
+    x = full["x"].astype(np.float64)
+    keep = (x >= lo) & (x < hi)
+    sc = {k: (v[keep] if isinstance(v, np.ndarray) and v.shape[:1] == x.shape else v) for k, v in full.items()}
+    sc["owned"] = (x[keep] >= x0) & (x[keep] < x1)
+    cell = float(full["sdf"]["cell"])
+    dims = (int(np.ceil((hi - lo) / cell)) + 3, int(full["sdf"]["dims"][1]), int(full["sdf"]["dims"][2]))
+    sc["sdf"] = sphere_in_box_sdf((lo, 0.0, 0.0), (hi, channel[1], channel[2]), full["sphere"]["centre"],
+                                  full["sphere"]["radius"], dims, cell)
+    sc["domain_min"] = (lo, 0.0, 0.0)
+    sc["domain_max"] = (hi, channel[1], channel[2])
+    sc["slab"] = {"index": slab, "of": nslabs, "x": (x0, x1), "halo": halo, "domain_x": (lo, hi),
+                  "owned": int(sc["owned"].sum()), "halo_particles": int((~sc["owned"]).sum())}
+    sc["name"] = f"flow_past_sphere_slab_{slab}_of_{nslabs}"
+    return sc
+
+
 def sweep_scenes(n_scenes=256, n_min=2000, n_max=200000, seed=11, tank=(1.0, 0.25, 0.6),
                  sdf_nodes=(64, 24, 40)):
     """Batched parameter sweep: dam-break scenes with particle counts
