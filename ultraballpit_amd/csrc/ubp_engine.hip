@@ -189,6 +189,7 @@ struct Engine {
   int pp = 0;
   uint32_t *key = nullptr, *arr = nullptr, *cnt = nullptr, *start = nullptr, *bsum = nullptr;
   uint32_t *stage_src = nullptr, *stage_orig = nullptr, *nbr = nullptr, *nn = nullptr, *nmask = nullptr;
+  float* p3 = nullptr;  // packed positions for the candidate test
   uint32_t* spread = nullptr;
   float* sdf = nullptr;
   float* dtp = nullptr;
@@ -264,7 +265,7 @@ struct Engine {
     dfree(rho_n); dfree(prs); dfree(dr); dfree(fr); dfree(xr);
     for (int k = 0; k < 2; ++k) { dfree(ft[k]); dfree(orig[k]); }
     dfree(key); dfree(arr); dfree(cnt); dfree(start); dfree(bsum); dfree(stage_src);
-    dfree(stage_orig); dfree(nbr); dfree(nn); dfree(nmask); dfree(spread); dfree(sdf); dfree(dtp); dfree(red);
+    dfree(stage_orig); dfree(nbr); dfree(p3); dfree(nn); dfree(nmask); dfree(spread); dfree(sdf); dfree(dtp); dfree(red);
     dfree(pmax); dfree(psum); dfree(dbg);
     dfree(bpos); dfree(bprs); dfree(bl_start); dfree(bl); dfree(bc_start); dfree(bc);
     dfree(d_sp); dfree(sid[0]); dfree(sid[1]); dfree(d_soff); dfree(sacc);
@@ -589,12 +590,17 @@ struct Engine {
   void alloc_nbr() {
     dfree(nbr);
     dalloc(&nbr, size_t(P.K) * n);  // flat neighbour lists, column-major [K][N]
+    if (!p3) {  // 4 slots of padding: the candidate test reads up to 3 past a cell's end
+      dalloc(&p3, 3 * (n + 4));
+      HIPCHK(hipMemsetAsync(p3, 0, 12 * (n + 4), st));
+    }
     if (!nmask) dalloc(&nmask, size_t(27) * n);  // per-cell masks [27][N]
   }
 
   // neighbour masks of the current positions, then the flat lists
   void build_neighbors(hipEvent_t* E = nullptr) {
-    UBP_LAUNCH_M(PASS_NEIGH, k_neigh, n, pos, start, nmask, nn);
+    launch(PASS_ELEM, k_pack3, n, (const float4*)pos, p3, uint32_t(n));
+    UBP_LAUNCH_M(PASS_NEIGH, k_neigh, n, pos, start, nmask, nn, (const float*)p3);
     rec(E, 7);
     if (n) {
       if (multi)

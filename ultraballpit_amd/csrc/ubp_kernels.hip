@@ -204,7 +204,9 @@ __global__ UBP_LB void k_rank_gather(DevParams P0, Scenes S, const uint32_t* __r
 // Neighbour structure, built in two passes each step:
 //  k_neigh  walks the 27 cells once (reference order: dz, dy, dx; slots
 //           ascending) and stores one 32-bit mask of accepted slots per cell,
-//           nmask[k * N + i] (column-major: every store coalesced);
+//           nmask[k * N + i] (column-major: every store coalesced); it reads
+//           candidates from a packed x y z copy of the positions (k_pack3),
+//           four slots per three 16-byte loads;
 //  k_expand turns the masks into flat lists nbr[t * N + i] through an LDS
 //           tile per wavefront, so the list rows are also stored coalesced.
 // The density, force and XSPH passes read the flat lists (one coalesced
@@ -381,10 +383,25 @@ __device__ __forceinline__ float3 ld_pos(__amdgpu_buffer_rsrc_t r, uint32_t lane
   return make_float3(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]));
 }
 
+// Positions packed as x y z (12 bytes per slot) for the candidate test.
+__global__ UBP_LB void k_pack3(const float4* __restrict__ pos, float* __restrict__ p3, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float4 p = pos[i];
+  p3[3u * i] = p.x;
+  p3[3u * i + 1u] = p.y;
+  p3[3u * i + 2u] = p.z;
+}
+
+__device__ __forceinline__ float4 ld_b128(__amdgpu_buffer_rsrc_t r, uint32_t lane_off, uint32_t uni_off) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, lane_off, uni_off, 0);
+  return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
+}
+
 template <bool M>
 __global__ UBP_LB void k_neigh(DevParams P0, Scenes S, const float4* __restrict__ pos,
                                const uint32_t* __restrict__ start, uint32_t* __restrict__ nmask,
-                               uint32_t* __restrict__ nn) {
+                               uint32_t* __restrict__ nn, const float* __restrict__ p3) {
   uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   uint32_t i = blk * blockDim.x + threadIdx.x;
   if (i >= P0.n) return;
@@ -397,6 +414,10 @@ __global__ UBP_LB void k_neigh(DevParams P0, Scenes S, const float4* __restrict_
   uint32_t c = 0, ovf = 0;
   uint32_t* m = nmask + i;
   const __amdgpu_buffer_rsrc_t prs = pos_rsrc(pos, P0.n);
+  // n + 4 packed positions: a trip reads up to 3 slots past a cell's end
+  const uint64_t p3bytes = uint64_t(P0.n + 4u) * 12u;
+  const __amdgpu_buffer_rsrc_t prs3 = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<float*>(p3), 0, int(p3bytes >= 0xffffffffull ? 0xffffffffu : uint32_t(p3bytes)), 0x00020000);
 #pragma unroll 1
   for (int k = 0; k < 27; ++k) {
     const int dz = k / 9, dy = (k / 3) % 3, dx = k % 3;
@@ -411,16 +432,30 @@ __global__ UBP_LB void k_neigh(DevParams P0, Scenes S, const float4* __restrict_
     const uint32_t lim = cnt < 32u ? cnt : 32u;
     const uint32_t boff = b * 16u;  // byte offset of the cell's first record
     uint32_t mask = 0u;
-    // t is the same in every active lane: the record offset t * 16 is a
-    // scalar operand; i's own slot is tested too (distance 0) and cleared
-    // below, and the count is the mask's popcount
-    for (uint32_t t = 0; t < lim; ++t) {
-      const float3 q = ld_pos(prs, boff, t * 16u);
-      const float ddx = pi.x - q.x;
-      const float ddy = pi.y - q.y;
-      const float ddz = pi.z - q.z;
-      const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-      mask = (d2 <= P.rsq) ? (mask | (1u << t)) : mask;
+    {
+      // four candidates per trip from three 16-byte loads of the packed
+      // positions (x y z, 12 bytes per slot); t is the same in every active
+      // lane, so the trip's byte offset is a scalar operand.  Candidates past
+      // the cell's end set bits that are cleared below; i's own slot is
+      // tested too (distance 0) and cleared after the walk.
+      const uint32_t off3 = b * 12u;
+      auto test = [&](float qx, float qy, float qz, uint32_t bit) {
+        const float ddx = pi.x - qx;
+        const float ddy = pi.y - qy;
+        const float ddz = pi.z - qz;
+        const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        mask = (d2 <= P.rsq) ? (mask | bit) : mask;
+      };
+      for (uint32_t t = 0; t < lim; t += 4u) {
+        const float4 A = ld_b128(prs3, off3, t * 12u);
+        const float4 B = ld_b128(prs3, off3, t * 12u + 16u);
+        const float4 C = ld_b128(prs3, off3, t * 12u + 32u);
+        test(A.x, A.y, A.z, 1u << t);
+        test(A.w, B.x, B.y, 2u << t);
+        test(B.z, B.w, C.x, 4u << t);
+        test(C.y, C.z, C.w, 8u << t);
+      }
+      mask &= (lim >= 32u) ? 0xffffffffu : ((1u << lim) - 1u);
     }
     if (k == 13 && i - b < 32u) mask &= ~(1u << (i - b));  // i itself (distance 0) is no neighbour
     c += uint32_t(__popc(mask));
