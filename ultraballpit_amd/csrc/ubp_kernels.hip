@@ -18,6 +18,10 @@
 namespace ubp {
 
 #define UBP_LB __launch_bounds__(256)
+// Neighbour-list entries are written once and read once per pass: they move
+// as non-temporal accesses so that they do not evict the neighbour records
+// the passes reuse from L2.
+#define UBP_LLD(p) __builtin_nontemporal_load(p)
 
 // Many-scene mode (template flag M): every slot carries its scene id; the
 // scene's parameters and dt are looked up per lane (each scene's DevParams
@@ -321,8 +325,8 @@ __device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, fl
     float4 qa[UNROLL][NV];
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u) {
-      ja[u] = (uint32_t(u) < c) ? l[size_t(u) * stride] : 0u;
-      jb[u] = (uint32_t(UNROLL + u) < c) ? l[size_t(UNROLL + u) * stride] : 0u;
+      ja[u] = (uint32_t(u) < c) ? UBP_LLD(l + size_t(u) * stride) : 0u;
+      jb[u] = (uint32_t(UNROLL + u) < c) ? UBP_LLD(l + size_t(UNROLL + u) * stride) : 0u;
     }
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u)
@@ -338,7 +342,7 @@ __device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, fl
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) {
         const uint32_t tn = t + 2u * UNROLL + u;
-        jc[u] = (tn < c) ? l[size_t(tn) * stride] : 0u;
+        jc[u] = (tn < c) ? UBP_LLD(l + size_t(tn) * stride) : 0u;
       }
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) f(ja[u], qa[u][0], qa[u]);
@@ -356,7 +360,7 @@ __device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, fl
       const float4* r[UNROLL];
       float4 q[UNROLL];
 #pragma unroll
-      for (int u = 0; u < UNROLL; ++u) j[u] = l[size_t(t + u) * stride];
+      for (int u = 0; u < UNROLL; ++u) j[u] = UBP_LLD(l + size_t(t + u) * stride);
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) { r[u] = rec + size_t(STRIDE) * j[u]; q[u] = r[u][0]; }
 #pragma unroll
@@ -364,7 +368,7 @@ __device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, fl
     }
   }
   for (; t < c; ++t) {  // the last c mod UNROLL
-    const uint32_t j0 = l[size_t(t) * stride];
+    const uint32_t j0 = UBP_LLD(l + size_t(t) * stride);
     const float4* r0 = rec + size_t(STRIDE) * j0;
     f(j0, r0[0], r0);
   }
@@ -538,7 +542,7 @@ __global__ __launch_bounds__(64) void k_expand(DevParams P0, Scenes S, const flo
     const uint32_t rows = (tmax - r0 < UBP_EXPAND_CHUNK) ? tmax - r0 : UBP_EXPAND_CHUNK;
     if (i < P.n)
       for (uint32_t t = 0; t < rows; ++t)
-        if (r0 + t < cnt) l[size_t(r0 + t) * P.n] = tile[t][lane];
+        if (r0 + t < cnt) __builtin_nontemporal_store(tile[t][lane], l + size_t(r0 + t) * P.n);
     __syncthreads();
   }
 }
