@@ -153,6 +153,17 @@ struct Snapshot {
     if (multi) launch(pass, kern<true>, count, P, scn(), __VA_ARGS__);           \
     else launch(pass, kern<false>, count, P, scn(), __VA_ARGS__);                \
   } while (0)
+// ... and for the list readers, with the engine's list access (nt_lists)
+#define UBP_LAUNCH_MN(pass, kern, count, ...)                                              \
+  do {                                                                                     \
+    if (multi) {                                                                           \
+      if (nt_lists) launch(pass, kern<true, true>, count, P, scn(), __VA_ARGS__);          \
+      else launch(pass, kern<true, false>, count, P, scn(), __VA_ARGS__);                  \
+    } else {                                                                               \
+      if (nt_lists) launch(pass, kern<false, true>, count, P, scn(), __VA_ARGS__);         \
+      else launch(pass, kern<false, false>, count, P, scn(), __VA_ARGS__);                 \
+    }                                                                                      \
+  } while (0)
 
 // Launch-shape passes the autotuner picks a workgroup size for.
 enum { PASS_ELEM = 0, PASS_NEIGH = 1, PASS_DENS = 2, PASS_FORCE = 3, PASS_XSPH = 4 };
@@ -190,6 +201,11 @@ struct Engine {
   uint32_t *key = nullptr, *arr = nullptr, *cnt = nullptr, *start = nullptr, *bsum = nullptr;
   uint32_t *stage_src = nullptr, *stage_orig = nullptr, *nbr = nullptr, *nn = nullptr, *nmask = nullptr;
   float* p3 = nullptr;  // packed positions for the candidate test
+  // Neighbour lists as non-temporal accesses while the pass records (64 B
+  // per particle for the force pass) fit in 192 MB of the 256 MB
+  // Infinity Cache: measured +2% at 1M and 2M particles, -0.4% at 4.2M and
+  // -3% for a 10.3M-particle batch.
+  bool nt_lists = true;
   uint32_t* spread = nullptr;
   float* sdf = nullptr;
   float* dtp = nullptr;
@@ -337,6 +353,7 @@ struct Engine {
 
   // device state for n particles over P.ncodes codes; uploads the mirror
   void alloc_upload() {
+    nt_lists = size_t(n) * 64 <= (size_t(192) << 20);
     dalloc(&pos, n); dalloc(&pos_t, n); dalloc(&vel, n); dalloc(&vel_t, n); dalloc(&acc, n);
     dalloc(&wall, n); dalloc(&wall_t, n); dalloc(&dv, n);
     dalloc(&rho_n, n); dalloc(&prs, n);
@@ -603,14 +620,10 @@ struct Engine {
     UBP_LAUNCH_M(PASS_NEIGH, k_neigh, n, pos, start, nmask, nn, (const float*)p3);
     rec(E, 7);
     if (n) {
-      if (multi)
-        hipLaunchKernelGGL(k_expand<true>, dim3(uint32_t((n + 63) / 64)), dim3(64), 0, st, P, scn(),
-                           (const float4*)pos, (const uint32_t*)start, (const uint32_t*)nmask,
-                           (const uint32_t*)nn, nbr);
-      else
-        hipLaunchKernelGGL(k_expand<false>, dim3(uint32_t((n + 63) / 64)), dim3(64), 0, st, P, scn(),
-                           (const float4*)pos, (const uint32_t*)start, (const uint32_t*)nmask,
-                           (const uint32_t*)nn, nbr);
+      auto kx = multi ? (nt_lists ? k_expand<true, true> : k_expand<true, false>)
+                      : (nt_lists ? k_expand<false, true> : k_expand<false, false>);
+      hipLaunchKernelGGL(kx, dim3(uint32_t((n + 63) / 64)), dim3(64), 0, st, P, scn(), (const float4*)pos,
+                         (const uint32_t*)start, (const uint32_t*)nmask, (const uint32_t*)nn, nbr);
       LAUNCHCHK("k_expand");
     }
   }
@@ -714,10 +727,10 @@ struct Engine {
     rec(E, 6);
     build_neighbors(E);
     rec(E, 8);
-    UBP_LAUNCH_M(PASS_DENS, k_density, n, dr, vel, wall, ft[pp], start, nbr, nn, dtp, rho_n, prs, fr);
+    UBP_LAUNCH_MN(PASS_DENS, k_density, n, dr, vel, wall, ft[pp], start, nbr, nn, dtp, rho_n, prs, fr);
     if (P.nb) launch(PASS_ELEM, k_adami, P.nb, P, pos, rho_n, prs, orig[pp], start, bprs);
     rec(E, 9);
-    UBP_LAUNCH_M(PASS_FORCE, k_forces, n, fr, wall, ft[pp], start, nbr, nn, acc);
+    UBP_LAUNCH_MN(PASS_FORCE, k_forces, n, fr, wall, ft[pp], start, nbr, nn, acc);
     rec(E, 10);
   }
 
@@ -736,7 +749,7 @@ struct Engine {
     rec(E, 0);
     UBP_LAUNCH_M(PASS_ELEM, k_kick, n, vel, acc, pos, rho_n, dtp, xr);
     rec(E, 1);
-    UBP_LAUNCH_M(PASS_XSPH, k_xsph, n, xr, start, nbr, nn, dv);
+    UBP_LAUNCH_MN(PASS_XSPH, k_xsph, n, xr, start, nbr, nn, dv);
     rec(E, 2);
     UBP_LAUNCH_M(PASS_ELEM, k_drift, n, pos, vel, dv, dtp, pos_t, vel_t, wall_t, key, arr, cnt,
                  (const float4*)nullptr, (const float4*)nullptr);
@@ -1133,7 +1146,7 @@ struct Engine {
   }
 
   void sur_features(float tau, float* rho_out) {
-    launch(PASS_DENS, k_feat, n, P, (const float4*)sr, (const float4*)wall, (const uint8_t*)ft[pp],
+    launch(PASS_DENS, nt_lists ? k_feat<true> : k_feat<false>, n, P, (const float4*)sr, (const float4*)wall, (const uint8_t*)ft[pp],
            (const uint32_t*)start, (const uint32_t*)nbr, (const uint32_t*)nn, tau, inv_u(), 1.0f / gmag(), feat,
            rho_out);
   }

@@ -18,10 +18,15 @@
 namespace ubp {
 
 #define UBP_LB __launch_bounds__(256)
-// Neighbour-list entries are written once and read once per pass: they move
-// as non-temporal accesses so that they do not evict the neighbour records
-// the passes reuse from L2.
-#define UBP_LLD(p) __builtin_nontemporal_load(p)
+// Neighbour-list entries are written once per step and read once per pass.
+// With NT they move as non-temporal accesses, so that they do not evict the
+// neighbour records the passes reuse from the caches; the engine selects NT
+// while those records fit the cache (a size threshold, measured).
+template <bool NT>
+__device__ __forceinline__ uint32_t ld_list(const uint32_t* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
 
 // Many-scene mode (template flag M): every slot carries its scene id; the
 // scene's parameters and dt are looked up per lane (each scene's DevParams
@@ -306,7 +311,7 @@ __device__ __forceinline__ void walk27r(const DevParams& P, uint32_t i, float4 p
 //          together at the top of the trip;
 //  MODE 1  list entries two trips ahead and records one trip ahead, held in
 //          registers: the loads of the next trip overlap this trip's bodies.
-template <int STRIDE, int UNROLL = 2, int NV = STRIDE, int MODE = 0, class F>
+template <int STRIDE, int UNROLL = 2, int NV = STRIDE, int MODE = 0, bool NT = false, class F>
 __device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, float4 pi,
                                               const float4* __restrict__ rec,
                                               const uint32_t* __restrict__ start,
@@ -325,8 +330,8 @@ __device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, fl
     float4 qa[UNROLL][NV];
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u) {
-      ja[u] = (uint32_t(u) < c) ? UBP_LLD(l + size_t(u) * stride) : 0u;
-      jb[u] = (uint32_t(UNROLL + u) < c) ? UBP_LLD(l + size_t(UNROLL + u) * stride) : 0u;
+      ja[u] = (uint32_t(u) < c) ? ld_list<NT>(l + size_t(u) * stride) : 0u;
+      jb[u] = (uint32_t(UNROLL + u) < c) ? ld_list<NT>(l + size_t(UNROLL + u) * stride) : 0u;
     }
 #pragma unroll
     for (int u = 0; u < UNROLL; ++u)
@@ -342,7 +347,7 @@ __device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, fl
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) {
         const uint32_t tn = t + 2u * UNROLL + u;
-        jc[u] = (tn < c) ? UBP_LLD(l + size_t(tn) * stride) : 0u;
+        jc[u] = (tn < c) ? ld_list<NT>(l + size_t(tn) * stride) : 0u;
       }
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) f(ja[u], qa[u][0], qa[u]);
@@ -360,7 +365,7 @@ __device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, fl
       const float4* r[UNROLL];
       float4 q[UNROLL];
 #pragma unroll
-      for (int u = 0; u < UNROLL; ++u) j[u] = UBP_LLD(l + size_t(t + u) * stride);
+      for (int u = 0; u < UNROLL; ++u) j[u] = ld_list<NT>(l + size_t(t + u) * stride);
 #pragma unroll
       for (int u = 0; u < UNROLL; ++u) { r[u] = rec + size_t(STRIDE) * j[u]; q[u] = r[u][0]; }
 #pragma unroll
@@ -368,7 +373,7 @@ __device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, fl
     }
   }
   for (; t < c; ++t) {  // the last c mod UNROLL
-    const uint32_t j0 = UBP_LLD(l + size_t(t) * stride);
+    const uint32_t j0 = ld_list<NT>(l + size_t(t) * stride);
     const float4* r0 = rec + size_t(STRIDE) * j0;
     f(j0, r0[0], r0);
   }
@@ -483,7 +488,7 @@ __global__ UBP_LB void k_neigh(DevParams P0, Scenes S, const float4* __restrict_
 // consecutive slots per row.  Decoded bits are cleared from the masks, so a
 // round resumes where the previous one stopped.
 #define UBP_EXPAND_CHUNK 32
-template <bool M>
+template <bool M, bool NT>
 __global__ __launch_bounds__(64) void k_expand(DevParams P0, Scenes S, const float4* __restrict__ pos,
                                                const uint32_t* __restrict__ start,
                                                const uint32_t* __restrict__ nmask,
@@ -542,7 +547,10 @@ __global__ __launch_bounds__(64) void k_expand(DevParams P0, Scenes S, const flo
     const uint32_t rows = (tmax - r0 < UBP_EXPAND_CHUNK) ? tmax - r0 : UBP_EXPAND_CHUNK;
     if (i < P.n)
       for (uint32_t t = 0; t < rows; ++t)
-        if (r0 + t < cnt) __builtin_nontemporal_store(tile[t][lane], l + size_t(r0 + t) * P.n);
+        if (r0 + t < cnt) {
+          if constexpr (NT) __builtin_nontemporal_store(tile[t][lane], l + size_t(r0 + t) * P.n);
+          else l[size_t(r0 + t) * P.n] = tile[t][lane];
+        }
     __syncthreads();
   }
 }
@@ -550,7 +558,7 @@ __global__ __launch_bounds__(64) void k_expand(DevParams P0, Scenes S, const flo
 // Density summation + delta-SPH (old density) + SDF wall volume + Tait EOS.
 // sph.rs:112-160 (density), sph.rs:169-216 (diffusion), lib.rs:232-237,
 // eos.rs:39-47.
-template <bool M>
+template <bool M, bool NT>
 __global__ UBP_LB void k_density(DevParams P0, Scenes S, const float4* __restrict__ dr,
                                  const float4* __restrict__ vel, const float4* __restrict__ wall,
                                  const uint8_t* __restrict__ ft, const uint32_t* __restrict__ start,
@@ -567,7 +575,7 @@ __global__ UBP_LB void k_density(DevParams P0, Scenes S, const float4* __restric
   Kahan rho, dif;
   rho.init(pi.w * P.norm_h3);  // m_i W(0, h)
   dif.init(0.0f);
-  for_neighbors<2, 8>(P, i, pi, dr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
+  for_neighbors<2, 8, 2, 0, NT>(P, i, pi, dr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
     float dx = pi.x - q.x;
     float dy = pi.y - q.y;
     float dz = pi.z - q.z;
@@ -618,7 +626,7 @@ __global__ UBP_LB void k_density(DevParams P0, Scenes S, const float4* __restric
 
 // Pressure (sph.rs:322-417) + Monaghan viscosity (sph.rs:432-492) + gravity
 // (sph.rs:502-508) + SDF wall pressure and repulsion.
-template <bool M>
+template <bool M, bool NT>
 __global__ UBP_LB void k_forces(DevParams P0, Scenes S, const float4* __restrict__ fr,
                                 const float4* __restrict__ wall, const uint8_t* __restrict__ ft,
                                 const uint32_t* __restrict__ start, const uint32_t* __restrict__ nbr,
@@ -636,7 +644,7 @@ __global__ UBP_LB void k_forces(DevParams P0, Scenes S, const float4* __restrict
   Kahan fx, fy, fz, vx, vy, vz;
   fx.init(0.0f); fy.init(0.0f); fz.init(0.0f);
   vx.init(0.0f); vy.init(0.0f); vz.init(0.0f);
-  for_neighbors<4, 4, 3, 1>(P, i, pi, fr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
+  for_neighbors<4, 4, 3, 1, NT>(P, i, pi, fr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
     float dx = pi.x - q.x;
     float dy = pi.y - q.y;
     float dz = pi.z - q.z;
@@ -995,6 +1003,7 @@ __global__ UBP_LB void k_cfl_finalize(DevParams P, const uint32_t* __restrict__ 
 // distance / normal, the phase tau within the correction cycle and the
 // resolution (h in units of 1 cm).
 // rho_out (optional) receives the summation density rho0 s0.
+template <bool NT>
 __global__ UBP_LB void k_feat(DevParams P, const float4* __restrict__ sr, const float4* __restrict__ wall,
                               const uint8_t* __restrict__ ft, const uint32_t* __restrict__ start,
                               const uint32_t* __restrict__ nbr, const uint32_t* __restrict__ nn, float tau,
@@ -1010,7 +1019,7 @@ __global__ UBP_LB void k_feat(DevParams P, const float4* __restrict__ sr, const 
   float s0 = pi.w * ir0 * P.norm_h3;
   float s1 = 0.0f, s2 = 0.0f, s3 = 0.0f, s4 = 0.0f, s5 = 0.0f, s6 = 0.0f, s7 = 0.0f;
   float s8 = 0.0f, s9 = 0.0f, s10 = 0.0f, s11 = 0.0f, s12 = 0.0f, s13 = 0.0f;
-  for_neighbors<3, 4, 3, 1>(P, i, pi, sr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
+  for_neighbors<3, 4, 3, 1, NT>(P, i, pi, sr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
     const float dx = pi.x - q.x;
     const float dy = pi.y - q.y;
     const float dz = pi.z - q.z;
@@ -1319,7 +1328,7 @@ __global__ __launch_bounds__(64) void k_cfl_finalize_scenes(const DevParams* __r
 
 // XSPH correction (sph.rs:236-280) on the post-kick velocities, over the
 // neighbour masks of the last density pass (same positions).
-template <bool M>
+template <bool M, bool NT>
 __global__ UBP_LB void k_xsph(DevParams P0, Scenes S, const float4* __restrict__ xr,
                               const uint32_t* __restrict__ start, const uint32_t* __restrict__ nbr,
                               const uint32_t* __restrict__ nn, float4* __restrict__ dv) {
@@ -1331,7 +1340,7 @@ __global__ UBP_LB void k_xsph(DevParams P0, Scenes S, const float4* __restrict__
   const float4 vi = xr[2u * i + 1u];
   const float rho_i = vi.w;
   float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-  for_neighbors<2, 10>(P, i, pi, xr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
+  for_neighbors<2, 10, 2, 0, NT>(P, i, pi, xr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
     float dx = pi.x - q.x;
     float dy = pi.y - q.y;
     float dz = pi.z - q.z;
