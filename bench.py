@@ -265,28 +265,40 @@ def slab_bench(steps: int, warmup: int):
     one holding the sphere's downstream half) with the 3 x 2h halo on each
     side that a once-per-step exchange needs; graph-replayed device-dt steps.
     Reports the owned particles' rate (the rate one GPU contributes to an
-    8-GPU run, exchange excluded) and the halo's share of the work."""
+    8-GPU run, exchange excluded) and the halo's share of the work; the same
+    for the 2h halo of a design with two exchanges per step."""
+    def timed(sc):
+        e = ubp.HipKernel.from_scene(sc)
+        e.step_n(warmup, use_graph=True)
+        e.sync()
+        t0 = time.perf_counter()
+        e.step_n(steps, use_graph=True)
+        e.sync()
+        t1 = time.perf_counter()
+        ok = e.state_finite()
+        e.close()
+        return t1 - t0, ok
+
+    # the halo of a design with two exchanges per step (2h on each side)
+    sc2 = scenes.flow_past_sphere_slab(slab=2, nslabs=8, halo_h=2.0)
+    dt2, _ = timed(sc2)
+    s2 = sc2["slab"]
+    del sc2
     sc = scenes.flow_past_sphere_slab(slab=2, nslabs=8)
     s = sc["slab"]
     n = len(sc["x"])
-    e = ubp.HipKernel.from_scene(sc)
-    e.step_n(warmup, use_graph=True)
-    e.sync()
-    t0 = time.perf_counter()
-    e.step_n(steps, use_graph=True)
-    e.sync()
-    t1 = time.perf_counter()
-    finite = e.state_finite()
-    e.close()
-    ms = (t1 - t0) * 1e3 / steps
+    dt, finite = timed(sc)
     return {"workload": f"flow past a sphere, slab {s['index'] + 1} of {s['of']} along x "
                         f"(x in [{s['x'][0]:.3f}, {s['x'][1]:.3f}] m) plus a {1e3 * s['halo']:.2f} mm (6 h) halo "
                         "on each side; slab faces are walls in this one-GPU stand-in",
             "owned": s["owned"], "halo": s["halo_particles"], "particles": n, "steps": steps,
-            "ms_per_step": ms, "value": s["owned"] * steps / (t1 - t0),
+            "ms_per_step": dt * 1e3 / steps, "value": s["owned"] * steps / dt,
             "unit": "owned particle-steps/s on one GPU (exchange excluded)",
-            "all_particle_steps_per_s": n * steps / (t1 - t0), "halo_fraction": s["halo_particles"] / n,
-            "state_finite": finite}
+            "all_particle_steps_per_s": n * steps / dt, "halo_fraction": s["halo_particles"] / n,
+            "state_finite": finite,
+            "halo_2h": {"halo": s2["halo_particles"], "particles": s2["owned"] + s2["halo_particles"],
+                        "ms_per_step": dt2 * 1e3 / steps, "value": s2["owned"] * steps / dt2,
+                        "halo_fraction": s2["halo_particles"] / (s2["owned"] + s2["halo_particles"])}}
 
 
 def main():
