@@ -615,8 +615,10 @@ struct Engine {
   }
 
   // neighbour masks of the current positions, then the flat lists
-  void build_neighbors(hipEvent_t* E = nullptr) {
-    launch(PASS_ELEM, k_pack3, n, (const float4*)pos, p3, uint32_t(n));
+  // (the binning's gather writes the packed positions p3; `repack` rebuilds
+  // them from pos where the positions came from elsewhere, a restore)
+  void build_neighbors(hipEvent_t* E = nullptr, bool repack = false) {
+    if (repack) launch(PASS_ELEM, k_pack3, n, (const float4*)pos, p3, uint32_t(n));
     UBP_LAUNCH_M(PASS_NEIGH, k_neigh, n, pos, start, nmask, nn, (const float*)p3);
     rec(E, 7);
     if (n) {
@@ -719,7 +721,7 @@ struct Engine {
     rec(E, 5);
     UBP_LAUNCH_M(PASS_ELEM, k_rank_gather, n, stage_src, stage_orig, key, start, pos_t, vel_t, rho_n, wall_t,
                  ft[pp], pos, vel, dr, wall, ft[pp ^ 1], orig[pp ^ 1], sid[pp], sid[pp ^ 1],
-                 (const float4*)(sur_on ? aux[pp] : nullptr), aux[pp ^ 1], sr);
+                 (const float4*)(sur_on ? aux[pp] : nullptr), aux[pp ^ 1], sr, p3);
     pp ^= 1;
   }
 
@@ -1025,7 +1027,7 @@ struct Engine {
     cp(start, S.start, (size_t(P.ncodes) + 1) * 4);
     if (multi) cp(sid[pp], S.sid, n * 2);
     if (sur_on && S.aux) cp(aux[pp], S.aux, n * 16);
-    if (!needs_init) build_neighbors();  // neighbour lists of the restored state
+    if (!needs_init) build_neighbors(nullptr, true);  // neighbour lists of the restored state
     dt_valid = false;
     invalidate_graph();
     sync();

@@ -174,7 +174,7 @@ __global__ UBP_LB void k_rank_gather(DevParams P0, Scenes S, const uint32_t* __r
                                      uint8_t* __restrict__ ft_dst, uint32_t* __restrict__ orig_dst,
                                      const uint16_t* __restrict__ sid_src, uint16_t* __restrict__ sid_dst,
                                      const float4* __restrict__ aux_src, float4* __restrict__ aux_dst,
-                                     float4* __restrict__ sr) {
+                                     float4* __restrict__ sr, float* __restrict__ p3) {
   uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= P0.n) return;
   uint32_t s = stage_src[d];
@@ -194,6 +194,9 @@ __global__ UBP_LB void k_rank_gather(DevParams P0, Scenes S, const uint32_t* __r
   float rd = rho;
   if (P.diff_floor > 0.0f && !(rd >= P.diff_floor)) rd = P.diff_floor;
   pos[t] = p;
+  p3[3u * t] = p.x;  // packed copy for the candidate test
+  p3[3u * t + 1u] = p.y;
+  p3[3u * t + 2u] = p.z;
   vel[t] = vel_t[s];
   dr[2u * t] = p;                                          // density record:
   dr[2u * t + 1u] = make_float4(rho, p.w / rd, 0.0f, 0.0f);  // (x y z m)(rho_old m/rho_old)
