@@ -111,6 +111,46 @@ def test_list_capacity_below_neighbour_counts_bitwise():
     assert_bitwise(e.particles(), o, "capacity 64 after 12 steps")
 
 
+def test_particles_on_cell_faces_and_outside_the_domain_bitwise():
+    """Positions exactly on cell faces (x and y at whole multiples of the FP32
+    cell size 2h from the domain minimum, so the binning's floor division
+    lands on the face), on the domain maximum and beyond the domain (binned
+    into the edge cells, then clamped): keys, permutation, neighbour counts
+    and state bitwise against the oracle after 10 steps."""
+    sc = scenes.water_box(n_side=10, dx=1e-3, box=(0.01, 0.01, 0.0125), sdf_nodes=48, with_sdf=False)
+    f32 = np.float32
+    cell = f32(2.0) * f32(sc["h"])
+    snapped = 0
+    for a in ("x", "y"):
+        v = sc[a].astype(np.float32)
+        k = np.rint(v / cell)
+        face = (k * cell).astype(np.float32)
+        near = (np.abs(v - face) < f32(3e-4)) & (k > 0)
+        v[near] = face[near]
+        snapped += int(near.sum())
+        sc[a] = v
+    assert snapped > 50
+    n = len(sc["x"])
+    # eight particles on and beyond the domain bounds (top layer, spread in x and y)
+    top = np.argsort(sc["z"])[-8:]
+    sc["z"][top[:3]] = f32(0.0125)              # on domain_max
+    sc["z"][top[3:5]] = f32(0.0131)             # beyond it
+    sc["x"][top[5]] = f32(0.0103)               # beyond x_max
+    sc["x"][top[6]] = f32(-0.0004)              # below x_min
+    sc["y"][top[7]] = f32(0.01)                 # on y_max
+    say(f"case: water box {n} particles, {snapped} snapped to cell faces, 8 on or outside the domain, 10 steps")
+    e = engine(sc)
+    o = oracle(sc)
+    e.step_n(10, use_graph=False)
+    o.step_auto(10)
+    ek, ep, ec = e.debug_bins()
+    ok, op, oc = o.bins(morton=True)
+    assert np.array_equal(ek, ok), "cell keys differ"
+    assert np.array_equal(ep, op), "sorted permutation differs"
+    assert np.array_equal(ec, oc), "neighbour counts differ"
+    assert_bitwise(e.particles(), o, "cell faces / domain edges after 10 steps")
+
+
 def test_config1_water_box_100_steps():
     say("case: water box 20^3 = 8000 particles, 100 KDK steps, CFL 0.25 device dt")
     sc = scenes.water_box()
