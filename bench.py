@@ -125,6 +125,45 @@ def cpu_baseline(scene, budget_s: float):
             "host": host_cpu()}
 
 
+def config1_leg():
+    """The 8,000-particle water box (100 device-dt steps) on the GPU, and the
+    same 100 steps of the CPU oracle in the reference's arithmetic on one
+    thread (faithful) and in the engine's arithmetic on all threads; the
+    final states of the engine and the engine-mode oracle are compared."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle  # noqa: E402  (checker only: CPU timings beside the GPU)
+    sc = scenes.water_box()
+    n = len(sc["x"])
+    e = ubp.HipKernel.from_scene(sc)
+    e.step_n(1, use_graph=False)  # bootstrap outside the timed steps
+    e.sync()
+    t0 = time.perf_counter()
+    e.step_n(99, use_graph=True)
+    e.sync()
+    t_gpu = time.perf_counter() - t0
+    p = e.particles()
+    e.close()
+    threads = pyoracle.set_threads(int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1)
+    out = {"workload": f"water box {n} particles, 10 x 10 x 12.5 mm, 100 steps", "particles": n,
+           "gpu": {"value": n * 99 / t_gpu, "unit": "particle-steps/s (steps 2-100, graph)"}}
+    for name, flags, thr in (("cpu_faithful_1_thread", pyoracle.FAITHFUL, 1), ("cpu_engine_all_threads",
+                                                                                pyoracle.ENGINE, threads)):
+        pyoracle.set_threads(thr)
+        o = pyoracle.Oracle(sc, precision=32, flags=flags)
+        o.step_auto(1)
+        t0 = time.perf_counter()
+        o.step_auto(99)
+        dt = time.perf_counter() - t0
+        out[name] = {"value": n * 99 / dt, "unit": "particle-steps/s (steps 2-100)", "threads": thr}
+        if flags == pyoracle.ENGINE:
+            out["engine_equals_oracle_bitwise"] = bool(all(
+                np.array_equal(np.asarray(p[k], np.float32).view(np.uint32),
+                               o.get(k).astype(np.float32).view(np.uint32)) for k in ("x", "y", "z", "vx", "vy", "vz",
+                                                                                      "density", "pressure")))
+    pyoracle.set_threads(threads)
+    return out
+
+
 def host_cpu() -> dict:
     """CPU model and the threads this process may run on."""
     model = None
@@ -410,6 +449,9 @@ def main():
     if args.sphere_steps > 0:
         log(f"[bench] flow past sphere, {args.sphere_steps} steps with force extraction")
         out["sphere"] = sphere_bench(args.sphere_steps)
+    if not args.no_cpu_baseline:
+        log("[bench] config 1 water box: GPU and CPU oracle, 100 steps")
+        out["config1"] = config1_leg()
     if args.slab_steps > 0:
         log(f"[bench] flow past sphere, slab 3 of 8 with halo, {args.slab_steps} steps")
         out["slab"] = slab_bench(args.slab_steps, min(args.warmup, 10))
