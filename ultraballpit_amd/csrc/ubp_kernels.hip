@@ -307,7 +307,9 @@ __device__ __forceinline__ void walk27r(const DevParams& P, uint32_t i, float4 p
 
 // Visit the listed neighbours of i in list order: f(j, pos_j, rec_j), where
 // rec_j points at the NV float4s of j's record that f may read.  Lists that
-// overflowed re-walk the cells (walk27r).  Two pipelines, chosen per pass by
+// overflowed re-walk the cells (walk27r, with i's scene parameters P); the
+// list capacity K and stride n come from PA, the kernel's argument block
+// (every scene of an engine shares them).  Two pipelines, chosen per pass by
 // measurement; the bodies run in list order either way, so the sums are the
 // same bits:
 //  MODE 0  UNROLL neighbours per trip, their list entries and records loaded
@@ -315,18 +317,18 @@ __device__ __forceinline__ void walk27r(const DevParams& P, uint32_t i, float4 p
 //  MODE 1  list entries two trips ahead and records one trip ahead, held in
 //          registers: the loads of the next trip overlap this trip's bodies.
 template <int STRIDE, int UNROLL = 2, int NV = STRIDE, int MODE = 0, bool NT = false, class F>
-__device__ __forceinline__ void for_neighbors(const DevParams& P, uint32_t i, float4 pi,
+__device__ __forceinline__ void for_neighbors(const DevParams& P, const DevParams& PA, uint32_t i, float4 pi,
                                               const float4* __restrict__ rec,
                                               const uint32_t* __restrict__ start,
                                               const uint32_t* __restrict__ nbr,
                                               const uint32_t* __restrict__ nn, F&& f) {
   const uint32_t c = nn[i];
-  if (c > P.K) {  // overflow bit set, or the list did not fit
+  if (c > PA.K) {  // overflow bit set, or the list did not fit
     walk27r<STRIDE>(P, i, pi, rec, start, f);
     return;
   }
   const uint32_t* l = nbr + i;
-  const size_t stride = P.n;
+  const size_t stride = PA.n;
   uint32_t t = 0;
   if constexpr (MODE == 1) {
     uint32_t ja[UNROLL], jb[UNROLL];
@@ -479,7 +481,7 @@ __global__ UBP_LB void k_neigh(DevParams P0, Scenes S, const float4* __restrict_
       const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
       if (d2 <= P.rsq && b + t != i) ++c;
     }
-    m[size_t(k) * P.n] = mask;
+    m[size_t(k) * P0.n] = mask;
   }
   nn[i] = c | ovf;
 }
@@ -506,9 +508,9 @@ __global__ __launch_bounds__(64) void k_expand(DevParams P0, Scenes S, const flo
   uint32_t cnt = 0;
 #pragma unroll
   for (int k = 0; k < 27; ++k) { mk[k] = 0u; bk[k] = 0u; }
-  if (i < P.n) {
+  if (i < P0.n) {
     const uint32_t nv = nn[i];
-    if (nv <= P.K) {
+    if (nv <= P0.K) {
       cnt = nv;
       const float4 pi = pos[i];
       uint32_t cx, cy, cz;
@@ -517,7 +519,7 @@ __global__ __launch_bounds__(64) void k_expand(DevParams P0, Scenes S, const flo
       nbhd(P, cx, cy, cz, N);
       const uint32_t* m = nmask + i;
 #pragma unroll
-      for (int k = 0; k < 27; ++k) mk[k] = m[size_t(k) * P.n];  // 27 loads in flight
+      for (int k = 0; k < 27; ++k) mk[k] = m[size_t(k) * P0.n];  // 27 loads in flight
 #pragma unroll
       for (int k = 0; k < 27; ++k) {
         const int dz = k / 9, dy = (k / 3) % 3, dx = k % 3;
@@ -548,11 +550,11 @@ __global__ __launch_bounds__(64) void k_expand(DevParams P0, Scenes S, const flo
     }
     __syncthreads();
     const uint32_t rows = (tmax - r0 < UBP_EXPAND_CHUNK) ? tmax - r0 : UBP_EXPAND_CHUNK;
-    if (i < P.n)
+    if (i < P0.n)
       for (uint32_t t = 0; t < rows; ++t)
         if (r0 + t < cnt) {
-          if constexpr (NT) __builtin_nontemporal_store(tile[t][lane], l + size_t(r0 + t) * P.n);
-          else l[size_t(r0 + t) * P.n] = tile[t][lane];
+          if constexpr (NT) __builtin_nontemporal_store(tile[t][lane], l + size_t(r0 + t) * P0.n);
+          else l[size_t(r0 + t) * P0.n] = tile[t][lane];
         }
     __syncthreads();
   }
@@ -578,7 +580,7 @@ __global__ UBP_LB void k_density(DevParams P0, Scenes S, const float4* __restric
   Kahan rho, dif;
   rho.init(pi.w * P.norm_h3);  // m_i W(0, h)
   dif.init(0.0f);
-  for_neighbors<2, 8, 2, 0, NT>(P, i, pi, dr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
+  for_neighbors<2, 8, 2, 0, NT>(P, P0, i, pi, dr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
     float dx = pi.x - q.x;
     float dy = pi.y - q.y;
     float dz = pi.z - q.z;
@@ -647,7 +649,7 @@ __global__ UBP_LB void k_forces(DevParams P0, Scenes S, const float4* __restrict
   Kahan fx, fy, fz, vx, vy, vz;
   fx.init(0.0f); fy.init(0.0f); fz.init(0.0f);
   vx.init(0.0f); vy.init(0.0f); vz.init(0.0f);
-  for_neighbors<4, 4, 3, 1, NT>(P, i, pi, fr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
+  for_neighbors<4, 4, 3, 1, NT>(P, P0, i, pi, fr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
     float dx = pi.x - q.x;
     float dy = pi.y - q.y;
     float dz = pi.z - q.z;
@@ -1022,7 +1024,7 @@ __global__ UBP_LB void k_feat(DevParams P, const float4* __restrict__ sr, const 
   float s0 = pi.w * ir0 * P.norm_h3;
   float s1 = 0.0f, s2 = 0.0f, s3 = 0.0f, s4 = 0.0f, s5 = 0.0f, s6 = 0.0f, s7 = 0.0f;
   float s8 = 0.0f, s9 = 0.0f, s10 = 0.0f, s11 = 0.0f, s12 = 0.0f, s13 = 0.0f;
-  for_neighbors<3, 4, 3, 1, NT>(P, i, pi, sr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
+  for_neighbors<3, 4, 3, 1, NT>(P, P, i, pi, sr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
     const float dx = pi.x - q.x;
     const float dy = pi.y - q.y;
     const float dz = pi.z - q.z;
@@ -1343,7 +1345,7 @@ __global__ UBP_LB void k_xsph(DevParams P0, Scenes S, const float4* __restrict__
   const float4 vi = xr[2u * i + 1u];
   const float rho_i = vi.w;
   float cx = 0.0f, cy = 0.0f, cz = 0.0f;
-  for_neighbors<2, 10, 2, 0, NT>(P, i, pi, xr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
+  for_neighbors<2, 10, 2, 0, NT>(P, P0, i, pi, xr, start, nbr, nn, [&](uint32_t, float4 q, const float4* rj) {
     float dx = pi.x - q.x;
     float dy = pi.y - q.y;
     float dz = pi.z - q.z;
