@@ -362,8 +362,9 @@ def main():
                                                               "dam_break_surrogate.npz"))
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the dam break would shard)", file=sys.stderr)
+        print("bench.py measures one GPU only: the engine has no multi-GPU path; the "
+              "'slab' entry of the one-GPU line times one GPU's share of a slab-sharded "
+              "scene", file=sys.stderr)
         sys.exit(2)
 
     if args.surrogate_only:
