@@ -11,6 +11,12 @@ namespace ubp {
 
 // Engine constants, computed once on the host (in FP32, with the same
 // expressions the reference evaluates per call) and passed by value.
+// Units per cell of the candidate test's 16-bit cell-relative coordinates:
+// a lane's coordinate relative to a neighbour cell lies in [-UBP_QS, 2 UBP_QS]
+// units, differences within +-2 UBP_QS, and 3 (2 UBP_QS + 66)^2 < 2^31, so the
+// squared distance of any candidate is exact in 32-bit integers.
+#define UBP_QS 12000
+
 struct DevParams {
   uint32_t n;          // particles
   uint32_t K;          // neighbour-list capacity per particle
@@ -37,6 +43,13 @@ struct DevParams {
   float b_tait[2];     // rho0 c^2 / gamma per fluid type (eos.rs:40)
   float rho0[2];       // 1000, 1.204 (eos.rs:7-10)
   float gamma[2];      // 7, 1.4 (eos.rs:13-19)
+  // integer pre-test of the candidate test: slot coordinates relative to the
+  // lower corner of their cell in units of cell / UBP_QS (qscale units per
+  // metre); squared unit distances <= qlo are neighbours, > qhi are not, the
+  // band between is decided by the FP32 test (quse = 0: FP32 test only)
+  float qscale;
+  int32_t qlo, qhi;
+  uint32_t quse;
   int sdf_on;
   float so[3];         // SDF origin
   float scell;         // SDF cell

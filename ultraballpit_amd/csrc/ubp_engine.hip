@@ -102,6 +102,37 @@ static void grid_params(DevParams& Q, uint32_t bits[3], float h, const float g[3
   Q.nb = 0;
 }
 
+// Thresholds of the candidate test's integer pre-test (k_neigh).  A slot's
+// coordinate is Q = rint((p - o) qscale), o = dmin + c cell the lower corner
+// of its cell, qscale = fl(UBP_QS / cell).  A lane tests cell C with its own
+// Q shifted by -delta UBP_QS (delta = C - own cell, per axis).  Per axis the
+// unit difference is off from (x_i - x_j) qscale by at most: 0.5 + 0.5 (the
+// two roundings to integers) + 4 ulp(L) qscale (two corners, each rounded
+// twice, L bounding the coordinates) + 0.01 (FP32 products and differences of
+// at most 2 UBP_QS units).  With ev = sqrt(3) e per vector, a squared unit
+// distance D'^2 differs from the exact one by at most ev (2 D + ev); the FP32
+// test's own rounding is < 1e-6 rsq.  qlo / qhi sit twice that below / above
+// rsq in units, so outside the band the integer test decides as the FP32 test
+// would.  Off (quse = 0) when the band exceeds 1% of rsq.
+static void set_qtest(DevParams& Q) {
+  const double S = UBP_QS;
+  Q.qscale = float(S / double(Q.cell));
+  double L = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    L = std::max(L, std::fabs(double(Q.dmin[a])));
+    L = std::max(L, std::fabs(double(Q.dmin[a]) + double(Q.dims[a]) * double(Q.cell)));
+    L = std::max(L, std::fabs(double(Q.dmax[a])));
+  }
+  const double ulpL = (L > 0.0) ? std::ldexp(1.0, std::ilogb(L) - 23) : std::ldexp(1.0, -149);
+  const double e = 1.0 + 4.0 * ulpL * double(Q.qscale) + 0.01;
+  const double ev = std::sqrt(3.0) * e;
+  const double rsq_u = double(Q.rsq) * double(Q.qscale) * double(Q.qscale);
+  const double band = 2.0 * (ev * (2.0 * std::sqrt(rsq_u) + ev) + 1e-6 * rsq_u);
+  Q.quse = (std::isfinite(rsq_u) && rsq_u > 0.0 && band < 0.01 * rsq_u && rsq_u + band < 1.5e9) ? 1u : 0u;
+  Q.qlo = Q.quse ? int32_t(std::floor(rsq_u - band)) : 0;
+  Q.qhi = Q.quse ? int32_t(std::ceil(rsq_u + band)) : 0;
+}
+
 // Constants derived from h, c_s, alpha, beta (sph.rs:20, :169-216, :432-492,
 // boundary.rs:143-181, eos.rs:39-47).
 static void set_derived(DevParams& Q, float alpha, float beta) {
@@ -127,6 +158,7 @@ static void set_derived(DevParams& Q, float alpha, float beta) {
     Q.gamma[f] = gam[f];
     Q.b_tait[f] = rho0[f] * Q.cs * Q.cs / gam[f];  // eos.rs:40
   }
+  set_qtest(Q);
 }
 
 // Device copy of the evolving state (checkpoint and autotuner trials).
@@ -201,6 +233,8 @@ struct Engine {
   uint32_t *key = nullptr, *arr = nullptr, *cnt = nullptr, *start = nullptr, *bsum = nullptr;
   uint32_t *stage_src = nullptr, *stage_orig = nullptr, *nbr = nullptr, *nn = nullptr, *nmask = nullptr;
   float* p3 = nullptr;  // packed positions for the candidate test
+  uint2* q16 = nullptr;  // 16-bit cell-relative coordinates for its integer pre-test
+  uint32_t* qbad = nullptr;  // [2]: a slot lay off its cell at the binning of parity pp
   // Neighbour lists as non-temporal accesses while the pass records (64 B
   // per particle for the force pass) fit in 192 MB of the 256 MB
   // Infinity Cache: measured +2% at 1M and 2M particles, -0.4% at 4.2M and
@@ -281,7 +315,7 @@ struct Engine {
     dfree(rho_n); dfree(prs); dfree(dr); dfree(fr); dfree(xr);
     for (int k = 0; k < 2; ++k) { dfree(ft[k]); dfree(orig[k]); }
     dfree(key); dfree(arr); dfree(cnt); dfree(start); dfree(bsum); dfree(stage_src);
-    dfree(stage_orig); dfree(nbr); dfree(p3); dfree(nn); dfree(nmask); dfree(spread); dfree(sdf); dfree(dtp); dfree(red);
+    dfree(stage_orig); dfree(nbr); dfree(p3); dfree(q16); dfree(qbad); dfree(nn); dfree(nmask); dfree(spread); dfree(sdf); dfree(dtp); dfree(red);
     dfree(pmax); dfree(psum); dfree(dbg);
     dfree(bpos); dfree(bprs); dfree(bl_start); dfree(bl); dfree(bc_start); dfree(bc);
     dfree(d_sp); dfree(sid[0]); dfree(sid[1]); dfree(d_soff); dfree(sacc);
@@ -610,6 +644,10 @@ struct Engine {
     if (!p3) {  // 4 slots of padding: the candidate test reads up to 3 past a cell's end
       dalloc(&p3, 3 * (n + 4));
       HIPCHK(hipMemsetAsync(p3, 0, 12 * (n + 4), st));
+      dalloc(&q16, n + 4);
+      HIPCHK(hipMemsetAsync(q16, 0, 8 * (n + 4), st));
+      dalloc(&qbad, 2);
+      HIPCHK(hipMemsetAsync(qbad, 0, 8, st));
     }
     if (!nmask) dalloc(&nmask, size_t(27) * n);  // per-cell masks [27][N]
   }
@@ -618,8 +656,14 @@ struct Engine {
   // (the binning's gather writes the packed positions p3; `repack` rebuilds
   // them from pos where the positions came from elsewhere, a restore)
   void build_neighbors(hipEvent_t* E = nullptr, bool repack = false) {
-    if (repack) launch(PASS_ELEM, k_pack3, n, (const float4*)pos, p3, uint32_t(n));
-    UBP_LAUNCH_M(PASS_NEIGH, k_neigh, n, pos, start, nmask, nn, (const float*)p3);
+    if (repack) {
+      // the 16-bit coordinates are not rebuilt: this list build takes the
+      // FP32 test, and the next binning's gather clears the word again
+      launch(PASS_ELEM, k_pack3, n, (const float4*)pos, p3, uint32_t(n));
+      launch(PASS_ELEM, k_set_word, 1, qbad + pp, 1u);
+    }
+    UBP_LAUNCH_M(PASS_NEIGH, k_neigh, n, pos, start, nmask, nn, (const float*)p3, (const uint2*)q16,
+                 (const uint32_t*)(qbad + pp));
     rec(E, 7);
     if (n) {
       auto kx = multi ? (nt_lists ? k_expand<true, true> : k_expand<true, false>)
@@ -721,7 +765,8 @@ struct Engine {
     rec(E, 5);
     UBP_LAUNCH_M(PASS_ELEM, k_rank_gather, n, stage_src, stage_orig, key, start, pos_t, vel_t, rho_n, wall_t,
                  ft[pp], pos, vel, dr, wall, ft[pp ^ 1], orig[pp ^ 1], sid[pp], sid[pp ^ 1],
-                 (const float4*)(sur_on ? aux[pp] : nullptr), aux[pp ^ 1], sr, p3);
+                 (const float4*)(sur_on ? aux[pp] : nullptr), aux[pp ^ 1], sr, p3, q16, qbad + (pp ^ 1),
+                 qbad + pp);
     pp ^= 1;
   }
 

@@ -174,8 +174,11 @@ __global__ UBP_LB void k_rank_gather(DevParams P0, Scenes S, const uint32_t* __r
                                      uint8_t* __restrict__ ft_dst, uint32_t* __restrict__ orig_dst,
                                      const uint16_t* __restrict__ sid_src, uint16_t* __restrict__ sid_dst,
                                      const float4* __restrict__ aux_src, float4* __restrict__ aux_dst,
-                                     float4* __restrict__ sr, float* __restrict__ p3) {
+                                     float4* __restrict__ sr, float* __restrict__ p3,
+                                     uint2* __restrict__ q16, uint32_t* __restrict__ qbad_set,
+                                     uint32_t* __restrict__ qbad_clear) {
   uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d == 0u) *qbad_clear = 0u;  // the word the next binning's gather sets
   if (d >= P0.n) return;
   uint32_t s = stage_src[d];
   uint32_t c = key[s];
@@ -197,6 +200,18 @@ __global__ UBP_LB void k_rank_gather(DevParams P0, Scenes S, const uint32_t* __r
   p3[3u * t] = p.x;  // packed copy for the candidate test
   p3[3u * t + 1u] = p.y;
   p3[3u * t + 2u] = p.z;
+  {  // 16-bit coordinates relative to the cell's lower corner (integer pre-test)
+    uint32_t cx, cy, cz;
+    pos_to_cell(P, p.x, p.y, p.z, cx, cy, cz);
+    const float qx = rintf((p.x - (P.dmin[0] + float(cx) * P.cell)) * P.qscale);
+    const float qy = rintf((p.y - (P.dmin[1] + float(cy) * P.cell)) * P.qscale);
+    const float qz = rintf((p.z - (P.dmin[2] + float(cz) * P.cell)) * P.qscale);
+    const float lo = -64.0f, hi = float(UBP_QS + 64);
+    const bool ok = qx >= lo && qx <= hi && qy >= lo && qy <= hi && qz >= lo && qz <= hi;  // false for NaN
+    const int32_t ix = ok ? int32_t(qx) : 0, iy = ok ? int32_t(qy) : 0, iz = ok ? int32_t(qz) : 0;
+    q16[t] = make_uint2((uint32_t(ix) & 0xffffu) | (uint32_t(iy) << 16), uint32_t(iz));
+    if (!ok && P.quse) atomicOr(qbad_set, 1u);  // some slot off its cell: FP32 test only
+  }
   vel[t] = vel_t[s];
   dr[2u * t] = p;                                          // density record:
   dr[2u * t + 1u] = make_float4(rho, p.w / rd, 0.0f, 0.0f);  // (x y z m)(rho_old m/rho_old)
@@ -216,9 +231,13 @@ __global__ UBP_LB void k_rank_gather(DevParams P0, Scenes S, const uint32_t* __r
 // Neighbour structure, built in two passes each step:
 //  k_neigh  walks the 27 cells once (reference order: dz, dy, dx; slots
 //           ascending) and stores one 32-bit mask of accepted slots per cell,
-//           nmask[k * N + i] (column-major: every store coalesced); it reads
-//           candidates from a packed x y z copy of the positions (k_pack3),
-//           four slots per three 16-byte loads;
+//           nmask[k * N + i] (column-major: every store coalesced).  It
+//           reads candidates as 16-bit cell-relative coordinates, two slots
+//           per 16-byte load, whose exact integer squared distance decides
+//           all but a thin band around 2h; the band (and every candidate,
+//           when the pre-test is off) takes the FP32 test on a packed x y z
+//           copy of the positions (k_pack3), four slots per three 16-byte
+//           loads;
 //  k_expand turns the masks into flat lists nbr[t * N + i] through an LDS
 //           tile per wavefront, so the list rows are also stored coalesced.
 // The density, force and XSPH passes read the flat lists (one coalesced
@@ -397,6 +416,11 @@ __device__ __forceinline__ float3 ld_pos(__amdgpu_buffer_rsrc_t r, uint32_t lane
   return make_float3(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]));
 }
 
+// One device word set by a kernel (a fill that graph capture may record).
+__global__ void k_set_word(uint32_t* w, uint32_t v) {
+  if (blockIdx.x == 0u && threadIdx.x == 0u) *w = v;
+}
+
 // Positions packed as x y z (12 bytes per slot) for the candidate test.
 __global__ UBP_LB void k_pack3(const float4* __restrict__ pos, float* __restrict__ p3, uint32_t n) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -412,15 +436,58 @@ __device__ __forceinline__ float4 ld_b128(__amdgpu_buffer_rsrc_t r, uint32_t lan
   return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
 }
 
+typedef short v2s16 __attribute__((ext_vector_type(2)));
+
+// Four candidates' outcomes shifted in at bit 0 of two masks:
+// a = 16 a + 8 [d0 <= lo] + 4 [d1 <= lo] + 2 [d2 <= lo] + [d3 <= lo], b the
+// same against hi.  Each comparison writes its own lane mask and one add with
+// carry-in (m + m + carry) shifts it in: one VALU operation per outcome.  The
+// comparisons come first, so seven instructions separate each mask's write
+// from its read (gfx950 needs two wait states between a VALU write of an
+// SGPR and a VALU read of it as carry-in).
+__device__ __forceinline__ void shift_in4x2(uint32_t& a, uint32_t& b, int32_t lo, int32_t hi, int32_t d0,
+                                            int32_t d1, int32_t d2, int32_t d3) {
+  uint64_t c0, c1, c2, c3, c4, c5, c6, c7;
+  asm("v_cmp_ge_i32_e64 %2, %10, %12\n\t"
+      "v_cmp_ge_i32_e64 %3, %11, %12\n\t"
+      "v_cmp_ge_i32_e64 %4, %10, %13\n\t"
+      "v_cmp_ge_i32_e64 %5, %11, %13\n\t"
+      "v_cmp_ge_i32_e64 %6, %10, %14\n\t"
+      "v_cmp_ge_i32_e64 %7, %11, %14\n\t"
+      "v_cmp_ge_i32_e64 %8, %10, %15\n\t"
+      "v_cmp_ge_i32_e64 %9, %11, %15\n\t"
+      "v_addc_co_u32_e64 %0, %2, %0, %0, %2\n\t"
+      "v_addc_co_u32_e64 %1, %3, %1, %1, %3\n\t"
+      "v_addc_co_u32_e64 %0, %4, %0, %0, %4\n\t"
+      "v_addc_co_u32_e64 %1, %5, %1, %1, %5\n\t"
+      "v_addc_co_u32_e64 %0, %6, %0, %0, %6\n\t"
+      "v_addc_co_u32_e64 %1, %7, %1, %1, %7\n\t"
+      "v_addc_co_u32_e64 %0, %8, %0, %0, %8\n\t"
+      "v_addc_co_u32_e64 %1, %9, %1, %1, %9"
+      : "+v"(a), "+v"(b), "=&s"(c0), "=&s"(c1), "=&s"(c2), "=&s"(c3), "=&s"(c4), "=&s"(c5), "=&s"(c6),
+        "=&s"(c7)
+      : "v"(lo), "v"(hi), "v"(d0), "v"(d1), "v"(d2), "v"(d3));
+}
+__device__ __forceinline__ uint4 ld_u128(__amdgpu_buffer_rsrc_t r, uint32_t lane_off, uint32_t uni_off) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, lane_off, uni_off, 0);
+  return make_uint4(v[0], v[1], v[2], v[3]);
+}
+__device__ __forceinline__ float3 ld_p3(__amdgpu_buffer_rsrc_t r, uint32_t lane_off) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, lane_off, 0, 0);
+  return make_float3(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]));
+}
+
 template <bool M>
 __global__ UBP_LB void k_neigh(DevParams P0, Scenes S, const float4* __restrict__ pos,
                                const uint32_t* __restrict__ start, uint32_t* __restrict__ nmask,
-                               uint32_t* __restrict__ nn, const float* __restrict__ p3) {
+                               uint32_t* __restrict__ nn, const float* __restrict__ p3,
+                               const uint2* __restrict__ q16, const uint32_t* __restrict__ qbad) {
   uint32_t blk = xcd_remap(blockIdx.x, gridDim.x);
   uint32_t i = blk * blockDim.x + threadIdx.x;
   if (i >= P0.n) return;
   UBP_SCENE(i);
   const float4 pi = pos[i];
+  const bool qtest = P.quse != 0u && *qbad == 0u;
   uint32_t cx, cy, cz;
   pos_to_cell(P, pi.x, pi.y, pi.z, cx, cy, cz);
   Nbhd N;
@@ -428,6 +495,20 @@ __global__ UBP_LB void k_neigh(DevParams P0, Scenes S, const float4* __restrict_
   uint32_t c = 0, ovf = 0;
   uint32_t* m = nmask + i;
   const __amdgpu_buffer_rsrc_t prs = pos_rsrc(pos, P0.n);
+  int32_t qix = 0, qiy = 0, qiz = 0;  // own 16-bit coordinates (sign-extended)
+  int32_t qlo_v, qhi_v;  // the thresholds in VGPRs, copied once
+  asm("" : "=v"(qlo_v) : "0"(P.qlo));
+  asm("" : "=v"(qhi_v) : "0"(P.qhi));
+  if (qtest) {
+    const uint2 qi = q16[i];
+    qix = int32_t(int16_t(qi.x & 0xffffu));
+    qiy = int32_t(int16_t(qi.x >> 16));
+    qiz = int32_t(qi.y);
+  }
+  // n + 4 16-bit coordinate records (8 bytes each)
+  const __amdgpu_buffer_rsrc_t q16rs = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<uint2*>(q16), 0, int(uint64_t(P0.n + 4u) * 8u >= 0xffffffffull ? 0xffffffffu : uint32_t(uint64_t(P0.n + 4u) * 8u)),
+      0x00020000);
   // n + 4 packed positions: a trip reads up to 3 slots past a cell's end
   const uint64_t p3bytes = uint64_t(P0.n + 4u) * 12u;
   const __amdgpu_buffer_rsrc_t prs3 = __builtin_amdgcn_make_buffer_rsrc(
@@ -446,7 +527,43 @@ __global__ UBP_LB void k_neigh(DevParams P0, Scenes S, const float4* __restrict_
     const uint32_t lim = cnt < 32u ? cnt : 32u;
     const uint32_t boff = b * 16u;  // byte offset of the cell's first record
     uint32_t mask = 0u;
-    {
+    if (qtest) {
+      // integer pre-test: two candidates per 16-byte load of the integer
+      // coordinates (x | y << 16 as 16-bit halves, z as 32 bits); the lane's own coordinates shifted to
+      // cell C's corner.  Each test shifts its outcomes in at bit 0 (bits
+      // reversed below); candidates in the band (qlo, qhi] are re-tested in
+      // FP32 after the walk.
+      const int32_t sx_ = int32_t(UBP_QS) * (dx - 1), sy_ = int32_t(UBP_QS) * (dy - 1),
+                    sz_ = int32_t(UBP_QS) * (dz - 1);
+      const uint32_t ixy = (uint32_t(qix - sx_) & 0xffffu) | (uint32_t(qiy - sy_) << 16);
+      const int32_t iz = qiz - sz_;
+      const uint32_t off8 = b * 8u;
+      uint32_t ma = 0u, mb = 0u;
+      auto qd2 = [&](uint32_t jxy, uint32_t jz) {
+        const v2s16 dxy = __builtin_bit_cast(v2s16, ixy) - __builtin_bit_cast(v2s16, jxy);
+        const int32_t dz = iz - int32_t(jz);  // |dz| < 2^23: exact 24-bit product
+        return __builtin_amdgcn_sdot2(dxy, dxy, __mul24(dz, dz), false);
+      };
+      for (uint32_t t = 0; t < lim; t += 4u) {
+        const uint4 A = ld_u128(q16rs, off8, t * 8u);
+        const uint4 B = ld_u128(q16rs, off8, t * 8u + 16u);
+        shift_in4x2(ma, mb, qlo_v, qhi_v, qd2(A.x, A.y), qd2(A.z, A.w), qd2(B.x, B.y), qd2(B.z, B.w));
+      }
+      const uint32_t L = (lim + 3u) & ~3u;  // tests made (<= 32)
+      const uint32_t keep = (lim >= 32u) ? 0xffffffffu : ((1u << lim) - 1u);
+      mask = (L == 0u) ? 0u : ((__builtin_bitreverse32(ma) >> (32u - L)) & keep);
+      uint32_t band = (L == 0u) ? 0u : ((__builtin_bitreverse32(mb) >> (32u - L)) & keep & ~mask);
+      while (band) {  // FP32 test of the band
+        const uint32_t t = uint32_t(__builtin_ctz(band));
+        band &= band - 1u;
+        const float3 q = ld_p3(prs3, (b + t) * 12u);
+        const float ddx = pi.x - q.x;
+        const float ddy = pi.y - q.y;
+        const float ddz = pi.z - q.z;
+        const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+        if (d2 <= P.rsq) mask |= 1u << t;
+      }
+    } else {
       // four candidates per trip from three 16-byte loads of the packed
       // positions (x y z, 12 bytes per slot); t is the same in every active
       // lane, so the trip's byte offset is a scalar operand.  Candidates past
