@@ -513,16 +513,26 @@ __global__ UBP_LB void k_neigh(DevParams P0, Scenes S, const float4* __restrict_
   const uint64_t p3bytes = uint64_t(P0.n + 4u) * 12u;
   const __amdgpu_buffer_rsrc_t prs3 = __builtin_amdgcn_make_buffer_rsrc(
       const_cast<float*>(p3), 0, int(p3bytes >= 0xffffffffull ? 0xffffffffu : uint32_t(p3bytes)), 0x00020000);
-#pragma unroll 1
-  for (int k = 0; k < 27; ++k) {
+  // slot range of cell k (0 past the grid); the next cell's range is loaded
+  // while this cell's candidates are tested
+  auto cell_range = [&](int k, uint32_t& b, uint32_t& e) {
     const int dz = k / 9, dy = (k / 3) % 3, dx = k % 3;
     const uint32_t sx = sel3(N.sx, dx), sy = sel3(N.sy, dy), sz = sel3(N.sz, dz);
-    uint32_t b = 0u, cnt = 0u;
+    b = 0u;
+    e = 0u;
     if (sx != 0xffffffffu && sy != 0xffffffffu && sz != 0xffffffffu) {
       const uint32_t code = P.code_base + (sx | sy | sz);
       b = start[code];
-      cnt = start[code + 1u] - b;
+      e = start[code + 1u];
     }
+  };
+  uint32_t nb_, ne_;
+  cell_range(0, nb_, ne_);
+#pragma unroll 1
+  for (int k = 0; k < 27; ++k) {
+    const int dz = k / 9, dy = (k / 3) % 3, dx = k % 3;
+    const uint32_t b = nb_, cnt = ne_ - nb_;
+    if (k < 26) cell_range(k + 1, nb_, ne_);
     if (cnt > 32u) ovf = UBP_NN_OVF;
     const uint32_t lim = cnt < 32u ? cnt : 32u;
     const uint32_t boff = b * 16u;  // byte offset of the cell's first record
