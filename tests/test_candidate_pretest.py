@@ -126,11 +126,11 @@ def test_pretest_outside_band_equals_fp32_test():
 
 
 def test_pretest_band_is_thin_for_the_benchmark_scenes():
-    # candidates re-tested in FP32: a band of well under 0.1% of (2h)^2
+    # candidates re-tested in FP32: a band at most 0.25% of (2h)^2 wide
     for h, dmin, dmax in SCENES[:3]:
         Q = set_qtest(h, dmin, dmax)
         rsq_u = float(Q["rsq"]) * float(Q["qscale"]) ** 2
-        assert (Q["qhi"] - Q["qlo"]) / rsq_u < 2e-3
+        assert (Q["qhi"] - Q["qlo"]) / rsq_u < 2.5e-3
 
 
 def test_pretest_off_when_coordinates_dwarf_the_cell():
