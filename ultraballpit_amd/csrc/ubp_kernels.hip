@@ -185,7 +185,13 @@ __global__ UBP_LB void k_rank_gather(DevParams P0, Scenes S, const uint32_t* __r
   uint32_t b = start[c], e = start[c + 1u];
   uint32_t me = stage_orig[d];
   uint32_t rank = 0;
-  for (uint32_t k = b; k < e; ++k) rank += (stage_orig[k] < me) ? 1u : 0u;
+  uint32_t k = b;
+  for (; k + 4u <= e; k += 4u) {  // four independent loads in flight per trip
+    const uint32_t o0 = stage_orig[k], o1 = stage_orig[k + 1u], o2 = stage_orig[k + 2u],
+                   o3 = stage_orig[k + 3u];
+    rank += uint32_t(o0 < me) + uint32_t(o1 < me) + uint32_t(o2 < me) + uint32_t(o3 < me);
+  }
+  for (; k < e; ++k) rank += (stage_orig[k] < me) ? 1u : 0u;
   uint32_t t = b + rank;
   const uint32_t sc_ = M ? uint32_t(sid_src[s]) : 0u;
   const DevParams& P = M ? S.sp[sc_] : P0;
