@@ -545,10 +545,10 @@ __global__ UBP_LB void k_neigh(DevParams P0, Scenes S, const float4* __restrict_
     uint32_t mask = 0u;
     if (qtest) {
       // integer pre-test: two candidates per 16-byte load of the integer
-      // coordinates (x | y << 16 as 16-bit halves, z as 32 bits); the lane's own coordinates shifted to
-      // cell C's corner.  Each test shifts its outcomes in at bit 0 (bits
-      // reversed below); candidates in the band (qlo, qhi] are re-tested in
-      // FP32 after the walk.
+      // coordinates (x | y << 16 as 16-bit halves, z as 32 bits), the
+      // lane's own coordinates shifted to cell C's corner.  Each test shifts
+      // its outcomes in at bit 0 (bits reversed below); candidates in the
+      // band (qlo, qhi] are re-tested in FP32 after the walk.
       const int32_t sx_ = int32_t(UBP_QS) * (dx - 1), sy_ = int32_t(UBP_QS) * (dy - 1),
                     sz_ = int32_t(UBP_QS) * (dz - 1);
       const uint32_t ixy = (uint32_t(qix - sx_) & 0xffffu) | (uint32_t(qiy - sy_) << 16);
